@@ -1,0 +1,107 @@
+/* C-ABI of the MI355X uniform-grid step (libwadi2d_hip.so).
+ *
+ * One handle holds one model's device arrays, as one backend object of the reference does
+ * (wadi2d/solver/gpu/backend.py:53-203, the protocol shared with solver/cpu/backend.py:14-141). Every entry point
+ * returns an int status: 0 ok, 1 invalid argument, 2 out of memory, 3 device error; w2d_last_error() gives the
+ * message of the last failure on the calling thread. Host buffers are plain pointers with element counts.
+ *
+ * Arrays cross this boundary in the reference's state layout (wadi2d/solver/layout.py:1-52): tile-major,
+ * 32 x 32 own cells inside a 2-cell halo, STRIDE 36, STRIDE2 1296, tile 0 the void tile. Element type is the
+ * handle's precision: float for W2D_F32, double for W2D_F64. Static tables cross in GridTables' tile layout
+ * (wadi2d/tables/builder.py:83-114), float32.
+ */
+#ifndef WADI2D_HIP_H
+#define WADI2D_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct w2d_handle w2d_handle;
+
+/* GridTables (builder.py:83-114) plus TileGrid.nbr (grid/tiles.py:30-35); (ntiles, 32, 32[, rows[, 2]]). */
+typedef struct w2d_tables {
+    int32_t ntiles;
+    int32_t rows;
+    const int32_t* nbr; /* (ntiles, 8) neighbour tile ids in TileGrid.NEIGHBOURS order */
+    const float* hc;
+    const float* dtab;
+    const float *flx, *frx, *glx, *grx, *efx, *cwx, *ftabx;
+    const float *fly, *fry, *gly, *gry, *efy, *cwy, *ftaby;
+    const float *zcx, *zcy;
+} w2d_tables;
+
+/* NumericsRecord (wadi2d/solver/backend.py:14-38); the kernels' Params (gpu/kernels/common.cuh:5-18). */
+typedef struct w2d_params {
+    double g, dx, e_wet, h_cfl, u_max, lead, a_min;
+    int32_t second, sheet, mirror, carry;
+} w2d_params;
+
+enum { W2D_F32 = 4, W2D_F64 = 8 };
+/* W2D_EXACT: IEEE division, square root and cube root, no contraction: the float32 statement order of
+ * solver/cpu/kernels.py. W2D_FAST: hardware reciprocal, square root and log/exp cube root in float32. */
+enum { W2D_EXACT = 0, W2D_FAST = 1 };
+
+/* Field ids: layout.STATE_ARRAYS then layout.WORK_ARRAYS (layout.py:25-29). */
+enum {
+    W2D_H, W2D_D, W2D_U, W2D_V, W2D_MX, W2D_MY, W2D_QX, W2D_QY,
+    W2D_U1, W2D_V1, W2D_MX1, W2D_MY1, W2D_QX1, W2D_QY1, W2D_R, W2D_SRC, W2D_RES, W2D_OM,
+    W2D_NFIELDS
+};
+
+/* Backend constructor (GpuBackend.__init__, gpu/backend.py:58-104): uploads every table. */
+int w2d_create(const w2d_tables* tables, const w2d_params* params, int32_t precision, int32_t variant,
+               w2d_handle** out);
+int w2d_destroy(w2d_handle* h);
+
+/* set_field / get_field (gpu/backend.py:121-125): n = ntiles * 1296 entries of the handle's type. get_field fills
+ * every halo entry from the tile that owns the cell, as the reference's exchange leaves them. */
+int w2d_set_field(w2d_handle* h, int32_t field, const void* host, int64_t n);
+int w2d_get_field(w2d_handle* h, int32_t field, void* host, int64_t n);
+/* Model.read_cells (model.py:139-143): entries at own-cell state indices. */
+int w2d_read_cells(w2d_handle* h, int32_t field, const int64_t* index, int64_t n, void* out);
+
+/* set_active (gpu/backend.py:116-119). */
+int w2d_set_active(w2d_handle* h, const int32_t* tiles, int32_t n);
+/* set_boundary (gpu/backend.py:127-132), BoundaryCells (solver/backend.py:41-55) at state indices. */
+int w2d_set_boundary(w2d_handle* h, const int32_t* cell, const int32_t* ref, const int32_t* kind, const float* d0,
+                     const float* h0, int32_t n);
+/* step (gpu/backend.py:151-188): faces, positivity, update, boundary; then the face buffers swap. */
+int w2d_step(w2d_handle* h, double dt, int32_t want_wave, int32_t want_wet);
+/* max_wave (gpu/backend.py:196-197). */
+int w2d_max_wave(w2d_handle* h, double* out);
+/* The per-tile wave speeds of the last wave step (the backend's `wave`, ntiles entries). */
+int w2d_wave(w2d_handle* h, double* out);
+/* wet_flags (gpu/backend.py:199-200) and the host's write of them (model.py:53). */
+int w2d_wet_flags(w2d_handle* h, int32_t* out);
+int w2d_set_wet(w2d_handle* h, const int32_t* flags);
+/* refresh_plan / refresh_halos (gpu/backend.py:139-149). No halos are stored on the device, so the second has
+ * nothing to do; it is kept for the protocol. */
+int w2d_refresh_plan(w2d_handle* h, const int32_t* tiles, int32_t n);
+int w2d_refresh_halos(w2d_handle* h);
+/* Tally (engine/maxima.py:149-192): FP64 volume through a face list, every face its own running total. */
+int w2d_tally_set(w2d_handle* h, const int32_t* face, const int32_t* axis, const float* sign, int32_t n);
+int w2d_tally_add(w2d_handle* h, double dt);
+int w2d_tally_volume(w2d_handle* h, double* out);
+/* The instantaneous flow through the tally's faces, sum of sign * q, m3/s (no counterpart: the outfall discharge). */
+int w2d_tally_flow(w2d_handle* h, double* out);
+/* Maxima (engine/maxima.py:28-51,103-146): depth, mean depth, speed, hazard, arrival; which = 0..4. */
+int w2d_maxima_set(w2d_handle* h, double arrival_depth, double speed_depth);
+int w2d_maxima_update(w2d_handle* h, double t);
+int w2d_maxima_get(w2d_handle* h, int32_t which, void* host, int64_t n);
+/* Monitor.storage (engine/monitor.py:214-218) without the dx^2: sum of h + res over listed own cells, FP64. */
+int w2d_storage_set(w2d_handle* h, const int64_t* index, int64_t n);
+int w2d_storage(w2d_handle* h, double* out);
+/* synchronize (gpu/backend.py:202-203). */
+int w2d_synchronize(w2d_handle* h);
+const char* w2d_last_error(void);
+/* Devices the HIP runtime sees (select_backend's probe, solver/backend.py:58-75). */
+int w2d_device_count(int32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,1300 @@
+// The uniform-grid shallow-water step on CDNA4 (gfx950), and its C-ABI (include/wadi2d_hip.h).
+//
+// Scheme: Wadi2D's sub-grid finite-volume step (wadi2d/solver/gpu/kernels/faces.cu, cells.cu, common.cuh; the
+// statement order of its Numba twin wadi2d/solver/cpu/kernels.py). The design here is this library's own:
+//
+//  * Device layout. Tiles of 32 x 32 own cells, tile-major, and no halo entries: cell (t, j, i) is at
+//    t * 1024 + j * 32 + i in every state array. Tile 0 is the void tile, all zero and never written. A kernel
+//    reads the 2-cell ring around its tile straight from the 8 neighbour tiles into LDS, so no exchange kernels
+//    and no halo traffic in HBM beyond the ring reads themselves.
+//  * Two kernels per step. `faces_kernel` computes both own faces of every own cell (velocity, momentum,
+//    discharge) and the tile's wave speed. `cells_kernel` recomputes the positivity ratio r on the tile and a
+//    one-cell ring in LDS (the reference's `limit` kernel and its exchange) and applies continuity, rain and the
+//    compensated residual, the level table inversion, the wet share and the tile's wet flag.
+//  * Tables. Face tables are plane-major records of consecutive row pairs, float4 (A_r, T_r, A_r+1, T_r+1) at
+//    r * ncell + cell, so a lookup is one 16-byte load and neighbouring lanes at the same row read one run of
+//    memory. Cell tables likewise, float2 (d_r, d_r+1). Face statics are packed float4 (fl, fr, gl, gr) per
+//    direction and (efx, cwx, efy, cwy).
+//  * Mean depth is double-buffered: the cells kernel reads the ring of a neighbour tile that another workgroup
+//    updates in the same launch.
+//
+// Arithmetic variants: R = double (the FP64 path), R = float exact (IEEE division/sqrt, cube root rounded once
+// from double, no contraction: the reference's float32 statement order), R = float fast (hardware rcp/sqrt and
+// a log2/exp2 cube root).
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "wadi2d_hip.h"
+
+#pragma clang fp contract(off)
+
+namespace {
+
+constexpr int TILE = 32;
+constexpr int TILE2 = TILE * TILE;
+constexpr int HALO = 2;
+constexpr int STRIDE = TILE + 2 * HALO;  // 36: the reference's state row, and the LDS row here
+constexpr int STRIDE2 = STRIDE * STRIDE;
+constexpr int NT = 256;                   // threads per workgroup: one tile
+constexpr int ROWS_PER_THREAD = TILE2 / NT;
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+    g_error = msg;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                                  \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess)                                                                          \
+            return fail(e_ == hipErrorOutOfMemory ? 2 : 3, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// ------------------------------------------------------------------------------------------------------------
+// arithmetic
+
+template <bool FAST>
+__device__ __forceinline__ float dv(float a, float b) {
+    if constexpr (FAST) return a * __builtin_amdgcn_rcpf(b);
+    return a / b;
+}
+template <bool FAST>
+__device__ __forceinline__ double dv(double a, double b) { return a / b; }
+
+template <bool FAST>
+__device__ __forceinline__ float sq(float x) {
+    if constexpr (FAST) return __builtin_amdgcn_sqrtf(x);
+    return sqrtf(x);
+}
+template <bool FAST>
+__device__ __forceinline__ double sq(double x) { return sqrt(x); }
+
+template <bool FAST>
+__device__ __forceinline__ float cb(float x) {
+    if constexpr (FAST) return x > 0.0f ? __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(x) * (1.0f / 3.0f)) : 0.0f;
+    return (float)cbrt((double)x);
+}
+template <bool FAST>
+__device__ __forceinline__ double cb(double x) { return cbrt(x); }
+
+template <class R>
+__device__ __forceinline__ R maxr(R a, R b) { return a > b ? a : (b > a ? b : a); }  // np.maximum-free: as max()
+template <class R>
+__device__ __forceinline__ R minr(R a, R b) { return a < b ? a : (b < a ? b : a); }
+
+// Python's max(a, b) returns a unless b > a; min(a, b) returns a unless b < a. The ternaries below keep that
+// order for the NaN cases the reference's float32 code can meet.
+template <class R>
+__device__ __forceinline__ R pmax(R a, R b) { return b > a ? b : a; }
+template <class R>
+__device__ __forceinline__ R pmin(R a, R b) { return b < a ? b : a; }
+
+// int(s) for the row of a table, capped; NaN and negative give row 0.
+template <class R>
+__device__ __forceinline__ int row_of(R s, int cap) {
+    if (!(s > R(0))) return 0;
+    return s < R(cap) ? (int)s : cap;
+}
+
+template <class R>
+__device__ __forceinline__ R minmod(R a, R b) {
+    if (a * b <= R(0)) return R(0);
+    return fabs(a) < fabs(b) ? a : b;
+}
+
+template <class R>
+__device__ __forceinline__ R ustar(R uc, R uup, R udown, bool second) {
+    if (!second) return uc;
+    return uc + R(0.5) * minmod(uc - uup, udown - uc);
+}
+
+template <class R>
+__device__ __forceinline__ R gated(R n, R own, bool mirror) { return (mirror && n == R(0)) ? own : n; }
+
+struct Params {
+    double g, dx, e_wet, h_cfl, u_max, lead, a_min;
+    int rows, second, sheet, mirror, carry;
+};
+
+// Area, conveyance^(3/5), wet width at depth e over the invert (common.cuh:64-90): rows quadratic in e, read
+// linearly between them, prismatic above the table top E. `tab` points at this face in plane 0; planes are
+// `plane` records apart.
+template <class R, bool FAST>
+__device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__ tab, size_t plane, int last, R& A,
+                                            R& T, R& W) {
+    const R m = R(last);
+    if (e >= E) {
+        const float4 q = tab[(size_t)(last - 1) * plane];
+        const R a1 = R(q.z), a0 = R(q.x);
+        const R c = dv<FAST>(m - R(1), m);
+        W = dv<FAST>(a1 - a0, E * (R(1) - c * c));
+        A = a1 + W * (e - E);
+        T = R(q.w) * dv<FAST>(A, pmax(a1, R(1e-12)));
+        return;
+    }
+    const R s = sq<FAST>(dv<FAST>(e, E)) * m;
+    const int r = row_of(s, last - 1);
+    const float4 q = tab[(size_t)r * plane];
+    const R c0 = dv<FAST>(R(r), m);
+    const R c1 = dv<FAST>(R(r + 1), m);
+    const R e0 = c0 * c0 * E;
+    const R de = (c1 * c1 - c0 * c0) * E;
+    const R f = dv<FAST>(e - e0, de);
+    const R a0 = R(q.x), a1 = R(q.z), t0 = R(q.y);
+    A = a0 + f * (a1 - a0);
+    T = t0 + f * (R(q.w) - t0);
+    W = dv<FAST>(a1 - a0, de);
+}
+
+// Level above zmin from the mean depth, and the wet share of the plan (common.cuh:39-56).
+template <class R, bool FAST>
+__device__ __forceinline__ R cell_level(R hb, R Hc, const float2* __restrict__ tab, size_t plane, int last) {
+    if (Hc <= R(0)) return hb;
+    if (hb >= Hc) return R(tab[(size_t)(last - 1) * plane].y) + (hb - Hc);
+    const R s = sq<FAST>(dv<FAST>(hb, Hc)) * R(last);
+    const int r = row_of(s, last - 1);
+    const float2 q = tab[(size_t)r * plane];
+    return R(q.x) + (s - R(r)) * (R(q.y) - R(q.x));
+}
+
+template <class R, bool FAST>
+__device__ __forceinline__ R cell_plan(R hb, R Hc, const float2* __restrict__ tab, size_t plane, int last) {
+    if (Hc <= R(0) || hb >= Hc) return R(1);
+    const R s = sq<FAST>(dv<FAST>(hb, Hc)) * R(last);
+    const int r = row_of(s, last - 1);
+    const float2 q = tab[(size_t)r * plane];
+    const R m = R(last);
+    return dv<FAST>(dv<FAST>(Hc * R(2 * r + 1), m * m), pmax(R(q.y) - R(q.x), R(1e-12)));
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// device arrays of one model
+
+template <class R>
+struct State {
+    R *h, *h_next, *d, *u, *v, *mx, *my, *u1, *v1, *mx1, *my1, *qx, *qy, *qx1, *qy1, *res, *src, *om;
+};
+
+struct Statics {
+    const float* hc;          // ncell
+    const float2* ctab;       // (rows - 1) planes of ncell
+    const float4* ftabx;      // (rows - 1) planes of ncell
+    const float4* ftaby;
+    const float4* sx;         // (fl, fr, gl, gr) of x faces
+    const float4* sy;
+    const float4* ec;         // (efx, cwx, efy, cwy)
+    const float* zcx;         // state layout with halos (ntiles * 1296), filled once
+    const float* zcy;
+    const int* nbr;           // (ntiles, 8)
+    size_t ncell;
+    int last;                 // rows - 1
+};
+
+// The 3 x 3 block of tiles around t, (dj + 1) * 3 + (di + 1); NEIGHBOURS order of grid/tiles.py:12.
+__device__ __forceinline__ void load_nb3(int t, const int* __restrict__ nbr, int* nb) {
+    if (threadIdx.x < 9) {
+        const int q = threadIdx.x;
+        nb[q] = q == 4 ? t : nbr[(size_t)t * 8 + (q < 4 ? q : q - 1)];
+    }
+}
+
+// Own cells and the 2-cell ring of a tile into LDS, STRIDE-wide rows (the reference's state layout, in LDS).
+template <class R>
+__device__ __forceinline__ void stage(R* __restrict__ lds, const R* __restrict__ g, const int* nb) {
+    for (int p = threadIdx.x; p < STRIDE2; p += NT) {
+        const int jj = p / STRIDE;
+        const int ii = p - jj * STRIDE;
+        const int j = jj - HALO, i = ii - HALO;
+        const int dj = j < 0 ? -1 : (j >= TILE ? 1 : 0);
+        const int di = i < 0 ? -1 : (i >= TILE ? 1 : 0);
+        const int tt = nb[(dj + 1) * 3 + di + 1];
+        lds[p] = g[(size_t)tt * TILE2 + (j - dj * TILE) * TILE + (i - di * TILE)];
+    }
+}
+
+// State index of the west / north neighbour of own cell (j, i) of tile t.
+__device__ __forceinline__ size_t west_of(int t, int j, int i, const int* nb) {
+    return i > 0 ? (size_t)t * TILE2 + j * TILE + i - 1 : (size_t)nb[3] * TILE2 + j * TILE + (TILE - 1);
+}
+__device__ __forceinline__ size_t north_of(int t, int j, int i, const int* nb) {
+    return j > 0 ? (size_t)t * TILE2 + (j - 1) * TILE + i : (size_t)nb[1] * TILE2 + (TILE - 1) * TILE + i;
+}
+
+template <class R>
+struct FaceIn {
+    const R* su;   // normal velocity (LDS)
+    const R* sv;   // transverse velocity (LDS)
+    const R* sq;   // normal discharge (LDS)
+    const R* sqp;  // transverse discharge (LDS)
+    const R* sh;
+    const R* sd;
+    int S, P;      // LDS steps along and across the normal
+};
+
+// One face: the west (x) or north (y) face of own cell L of the LDS block; faces.cu:105-242 restated, with the
+// statement order of solver/cpu/kernels.py:86-219. Returns the wave speed.
+template <class R, bool FAST>
+__device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st, R ef, R cw, const float* zc,
+                                  size_t zk, const float4* __restrict__ ftab, size_t plane, int last,
+                                  const R* __restrict__ om, size_t k_up, size_t k_own, R dt, bool want_wave,
+                                  const Params& P, R& u_out, R& m_out, R& q_out) {
+    const int S = in.S, Q = in.P;
+    const R* u = in.su;
+    const R g = R(P.g), dx = R(P.dx);
+    const R fl = R(st.x), fr = R(st.y), gl = R(st.z), gr = R(st.w);
+    const R eL = in.sd[L - S] - fl;
+    const R eR = in.sd[L] - fr;
+    const R uf = u[L];
+    const R eLp = pmax(eL, R(0));
+    const R eRp = pmax(eR, R(0));
+    const bool from_left = uf > R(0) || (uf == R(0) && eL >= eR);
+    const R eup = from_left ? eLp : eRp;
+    if (eup <= R(P.e_wet)) {
+        u_out = R(0);
+        m_out = R(0);
+        q_out = R(0);
+        return R(0);
+    }
+    R eface = eup;
+    if (P.carry && cw == R(0)) {
+        const R dz = from_left ? R(zc[zk - S]) - fl : R(zc[zk]) - fr;
+        if (dz > R(1e-6)) {
+            R phi = dv<FAST>(eup - R(2) * dz, R(2) * dz);
+            phi = phi < R(0) ? R(0) : (phi > R(1) ? R(1) : phi);
+            eface = eup - phi * dz;
+        }
+    }
+    R A, T, W;
+    face_lookup<R, FAST>(eface, ef, ftab, plane, last, A, T, W);
+    const R hL = in.sh[L - S];
+    const R hR = in.sh[L];
+    const R hup = from_left ? hL : hR;
+
+    // [1] section
+    R rho = R(1);
+    R dEta = eRp - eLp;
+    if (P.sheet) {
+        const R Avol = hup * dx;
+        const R edown = from_left ? eRp : eLp;
+        if (Avol < A) {
+            const R s = dv<FAST>(pmin(eLp, eRp), pmax(eLp, eRp));
+            const R s2 = s * s;
+            const R alpha = s2 * s2;
+            rho = alpha + dv<FAST>((R(1) - alpha) * Avol, A);
+        }
+        if (edown <= R(0)) {
+            // [2] disconnected
+            const R gU = from_left ? gl : gr;
+            const R gD = from_left ? gr : gl;
+            const R hD = from_left ? hR : hL;
+            const R drop = (hup - gU) - pmax(hD - gD, R(0.5) * (gU - gD));
+            const R level_drop = fabs(eL - eR);
+            const R drive = pmax(eup, pmin(drop, level_drop));
+            dEta = from_left ? -drive : drive;
+        }
+    }
+
+    // [3] upwinded momentum fluxes, along then across
+    const bool mirror = P.mirror, second = P.second;
+    const R um = gated(u[L - S], uf, mirror);
+    const R up = gated(u[L + S], uf, mirror);
+    const R umm = gated(u[L - 2 * S], um, mirror);
+    const R upp = gated(u[L + 2 * S], up, mirror);
+    const R* q = in.sq;
+    const R QcL = R(0.5) * (q[L - S] + q[L]);
+    const R QcR = R(0.5) * (q[L] + q[L + S]);
+    const R sL = QcL > R(0) ? ustar(um, umm, uf, second) : ustar(uf, up, um, second);
+    const R sR = QcR > R(0) ? ustar(uf, um, up, second) : ustar(up, upp, uf, second);
+    R flux = QcR * sR - QcL * sL;
+    const R* qp = in.sqp;
+    const R QyS = R(0.5) * (qp[L - S] + qp[L]);
+    const R QyN = R(0.5) * (qp[L - S + Q] + qp[L + Q]);
+    const R uS = gated(u[L - Q], uf, mirror);
+    const R uN = gated(u[L + Q], uf, mirror);
+    const R uSS = gated(u[L - 2 * Q], uS, mirror);
+    const R uNN = gated(u[L + 2 * Q], uN, mirror);
+    const R usS = QyS > R(0) ? ustar(uS, uSS, uf, second) : ustar(uf, uN, uS, second);
+    const R usN = QyN > R(0) ? ustar(uf, uS, uN, second) : ustar(uN, uNN, uf, second);
+    flux += QyN * usN - QyS * usS;
+
+    // [4] momentum and pressure
+    const R Abar = R(0.5) * (hL + hR) * dx;
+    const R press = dv<FAST>(dt * g * dEta, dx);
+    R us;
+    if (Abar > R(1e-6) * dx)
+        us = dv<FAST>(m_own - dv<FAST>(dt * flux, dx), Abar) - press;
+    else
+        us = uf - press;
+
+    // [5] friction, implicit in the speed
+    const R* vp = in.sv;
+    const R vb = R(0.25) * (vp[L - S] + vp[L] + vp[L - S + Q] + vp[L + Q]);
+    const R T3 = T * T * T * cb<FAST>(T);
+    R fric = dv<FAST>(A * A, pmax(T3, R(1e-30)));
+    if (rho < R(1)) fric = dv<FAST>(fric, pmax(rho * cb<FAST>(rho), R(1e-12)));
+    const R a = dt * g * fric;
+    const R sp = sq<FAST>(us * us + vb * vb);
+    R un = dv<FAST>(R(2) * us, R(1) + sq<FAST>(R(1) + R(4) * a * sp));
+    const R umax = R(P.u_max);
+    if (fabs(un) > umax) un = un > R(0) ? umax : -umax;
+
+    // [6] crest face: no more than critical flow over the crest
+    R qn = A * rho * un;
+    R c = cw;
+    if (c != R(0)) {
+        R Ac = A, Wc = W;
+        if (c < R(0)) {
+            c = -c;
+            const R ua = from_left ? um : up;
+            R Tc;
+            face_lookup<R, FAST>(eup + dv<FAST>(R(0.5) * ua * ua, g), ef, ftab, plane, last, Ac, Tc, Wc);
+        }
+        const R qc = c * Ac * sq<FAST>(dv<FAST>(Ac, pmax(Wc, R(1e-9))));
+        if (fabs(qn) > qc) {
+            qn = qn > R(0) ? qc : -qc;
+            un = dv<FAST>(qn, pmax(A * rho, R(1e-12)));
+        }
+    }
+    u_out = un;
+    m_out = Abar * un;
+    q_out = qn;
+    if (!want_wave) return R(0);
+    if (!isfinite(un)) return R(1e30);
+    // wave speed for the next refresh
+    const R dL = in.sd[L - S];
+    const R dR = in.sd[L];
+    const R dup = from_left ? dL : dR;
+    const R h_cfl = R(P.h_cfl);
+    if (dup <= h_cfl) return R(0);
+    R plan = from_left ? om[k_up] : om[k_own];
+    const R dother = from_left ? dR : dL;
+    if (dother > h_cfl) plan = pmin(plan, from_left ? om[k_own] : om[k_up]);
+    const R wave = sq<FAST>(pmax(dv<FAST>(g * A * rho, pmax(plan, R(P.a_min)) * dx), R(0)));
+    return fabs(un) + pmin(R(P.lead) * fabs(un - uf), R(2) * wave) + wave;
+}
+
+template <class R>
+__device__ __forceinline__ R block_max(R v, R* red) {
+    for (int off = 32; off > 0; off >>= 1) {
+        const R o = __shfl_xor(v, off);
+        v = o > v ? o : v;
+    }
+    const int wid = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) red[wid] = v;
+    __syncthreads();
+    R m = red[0];
+    for (int w = 1; w < NT / 64; ++w) m = red[w] > m ? red[w] : m;
+    return m;
+}
+
+template <class R>
+struct StepArgs {
+    State<R> s;
+    Statics st;
+    Params P;
+    const int* active;
+    int nact;
+    R dt;
+    int want_wave;
+    R* wave;     // per tile
+    int* wet;    // per tile
+};
+
+// Both own faces of every own cell of one active tile per workgroup (faces.cu:244-271).
+template <class R, bool FAST>
+__global__ __launch_bounds__(NT) void faces_kernel(StepArgs<R> A) {
+    __shared__ R s_u[STRIDE2], s_v[STRIDE2], s_h[STRIDE2], s_d[STRIDE2], s_qx[STRIDE2], s_qy[STRIDE2];
+    __shared__ int nb[9];
+    __shared__ R red[NT / 64];
+    if ((int)blockIdx.x >= A.nact) return;
+    const int t = A.active[blockIdx.x];
+    load_nb3(t, A.st.nbr, nb);
+    __syncthreads();
+    stage(s_u, A.s.u, nb);
+    stage(s_v, A.s.v, nb);
+    stage(s_h, A.s.h, nb);
+    stage(s_d, A.s.d, nb);
+    stage(s_qx, A.s.qx, nb);
+    stage(s_qy, A.s.qy, nb);
+    __syncthreads();
+    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, STRIDE};
+    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, STRIDE, 1};
+    const int i = threadIdx.x & (TILE - 1);
+    const bool want_wave = A.want_wave != 0;
+    R w = R(0);
+#pragma unroll 1
+    for (int r = 0; r < ROWS_PER_THREAD; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NT / TILE);
+        const int L = (j + HALO) * STRIDE + i + HALO;
+        const size_t c = (size_t)t * TILE2 + j * TILE + i;
+        const size_t zk = (size_t)t * STRIDE2 + L;
+        const float4 e = A.st.ec[c];
+        R un, mn, qn;
+        const R wx = face<R, FAST>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, zk, A.st.ftabx + c,
+                                   A.st.ncell, A.st.last, A.s.om, want_wave ? west_of(t, j, i, nb) : c, c, A.dt,
+                                   want_wave, A.P, un, mn, qn);
+        A.s.u1[c] = un;
+        A.s.mx1[c] = mn;
+        A.s.qx1[c] = qn;
+        const R wy = face<R, FAST>(fy, L, A.s.my[c], A.st.sy[c], R(e.z), R(e.w), A.st.zcy, zk, A.st.ftaby + c,
+                                   A.st.ncell, A.st.last, A.s.om, want_wave ? north_of(t, j, i, nb) : c, c, A.dt,
+                                   want_wave, A.P, un, mn, qn);
+        A.s.v1[c] = un;
+        A.s.my1[c] = mn;
+        A.s.qy1[c] = qn;
+        const R m = wx > wy ? wx : wy;
+        w = m > w ? m : w;
+    }
+    if (want_wave) {
+        const R m = block_max(w, red);
+        if (threadIdx.x == 0) A.wave[t] = m;
+    }
+}
+
+template <class R>
+__device__ __forceinline__ R scaled(R q, R rl, R rr) { return q * (q > R(0) ? rl : rr); }
+
+// Positivity ratio on the tile and a one-cell ring (cells.cu:4-18), then continuity, rain, the compensated
+// residual, the level and wet share of every own cell (cells.cu:28-63).
+template <class R, bool FAST>
+__global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
+    __shared__ R s_h[STRIDE2], s_qx[STRIDE2], s_qy[STRIDE2], s_r[STRIDE2];
+    __shared__ int nb[9];
+    __shared__ int any_wet;
+    if ((int)blockIdx.x >= A.nact) return;
+    const int t = A.active[blockIdx.x];
+    load_nb3(t, A.st.nbr, nb);
+    if (threadIdx.x == 0) any_wet = 0;
+    __syncthreads();
+    stage(s_h, A.s.h, nb);
+    stage(s_qx, A.s.qx1, nb);
+    stage(s_qy, A.s.qy1, nb);
+    __syncthreads();
+    const R dx = R(A.P.dx);
+    const R area = dx * dx;
+    const R dt = A.dt;
+    constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32
+    for (int p = threadIdx.x; p < RW * RW; p += NT) {
+        const int jj = p / RW, ii = p - (p / RW) * RW;
+        const int L = (jj + 1) * STRIDE + ii + 1;
+        const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + STRIDE], R(0)) +
+                      pmax(-s_qy[L], R(0));
+        const R have = pmax(s_h[L], R(0)) * area;
+        s_r[L] = out * dt > have ? dv<FAST>(have, out * dt) : R(1);
+    }
+    __syncthreads();
+    const int i = threadIdx.x & (TILE - 1);
+    const bool want_plan = A.want_wave != 0;
+    int wet = 0;
+#pragma unroll 1
+    for (int r = 0; r < ROWS_PER_THREAD; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NT / TILE);
+        const int L = (j + HALO) * STRIDE + i + HALO;
+        const size_t c = (size_t)t * TILE2 + j * TILE + i;
+        const R rc = s_r[L];
+        const R qw = scaled(s_qx[L], s_r[L - 1], rc);
+        const R qe = scaled(s_qx[L + 1], rc, s_r[L + 1]);
+        const R qn = scaled(s_qy[L], s_r[L - STRIDE], rc);
+        const R qs = scaled(s_qy[L + STRIDE], rc, s_r[L + STRIDE]);
+        A.s.qx[c] = qw;
+        A.s.qy[c] = qn;
+        const R h0 = s_h[L];
+        const R inc = A.s.res[c] + dt * A.s.src[c] - dv<FAST>(dt * (qe - qw + qs - qn), area);
+        const R sum = h0 + inc;
+        const R hn = pmax(sum, R(0));
+        A.s.res[c] = sum > R(0) ? inc - (sum - h0) : R(0);
+        A.s.h_next[c] = hn;
+        const R Hc = R(A.st.hc[c]);
+        A.s.d[c] = cell_level<R, FAST>(hn, Hc, A.st.ctab + c, A.st.ncell, A.st.last);
+        if (want_plan) A.s.om[c] = cell_plan<R, FAST>(hn, Hc, A.st.ctab + c, A.st.ncell, A.st.last);
+        wet |= hn > R(0);
+    }
+    if (wet) any_wet = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) A.wet[t] = any_wet;
+}
+
+// Wet share of the listed tiles' cells (cells.cu:66-77).
+template <class R, bool FAST>
+__global__ __launch_bounds__(NT) void plan_kernel(const int* __restrict__ tiles, int n, const R* __restrict__ h,
+                                                  R* __restrict__ om, const float* __restrict__ hc,
+                                                  const float2* __restrict__ ctab, size_t ncell, int last) {
+    if ((int)blockIdx.x >= n) return;
+    const int t = tiles[blockIdx.x];
+    for (int p = threadIdx.x; p < TILE2; p += NT) {
+        const size_t c = (size_t)t * TILE2 + p;
+        om[c] = cell_plan<R, FAST>(h[c], R(hc[c]), ctab + c, ncell, last);
+    }
+}
+
+// Boundary cells (cells.cu:95-111): kind 0 fixed, 1 free (dry), 2 follows its inner cell lowered by d0.
+template <class R>
+__global__ void boundary_kernel(int n, const int* __restrict__ cell, const int* __restrict__ ref,
+                                const int* __restrict__ kind, const float* __restrict__ d0,
+                                const float* __restrict__ h0, R* __restrict__ h, R* __restrict__ d) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const int k = cell[m];
+    if (kind[m] == 0) {
+        h[k] = R(h0[m]);
+        d[k] = R(d0[m]);
+    } else if (kind[m] == 1) {
+        h[k] = R(0);
+        d[k] = R(0);
+    } else {
+        h[k] = h[ref[m]];
+        d[k] = pmax(d[ref[m]] + R(d0[m]), R(0));
+    }
+}
+
+// Copy whole tiles between the two mean-depth buffers: a tile that leaves the active list keeps one value.
+template <class R>
+__global__ void copy_tiles_kernel(const int* __restrict__ tiles, int n, const R* __restrict__ a, R* __restrict__ b) {
+    if ((int)blockIdx.x >= n) return;
+    const size_t base = (size_t)tiles[blockIdx.x] * TILE2;
+    for (int p = threadIdx.x; p < TILE2; p += NT) b[base + p] = a[base + p];
+}
+
+// Max of the active tiles' wave speeds into out[0], by one workgroup.
+template <class R>
+__global__ __launch_bounds__(1024) void wave_max_kernel(const int* __restrict__ active, int n,
+                                                        const R* __restrict__ wave, double* __restrict__ out) {
+    __shared__ double red[16];
+    double m = 0.0;
+    for (int s = threadIdx.x; s < n; s += blockDim.x) {
+        const double v = (double)wave[active[s]];
+        m = v > m ? v : m;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(m, off);
+        m = o > m ? o : m;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) r = red[w] > r ? red[w] : r;
+        out[0] = r;
+    }
+}
+
+// One thread per face, each adding to its own FP64 total, no atomics (engine/maxima.py:53-62).
+template <class R>
+__global__ void tally_kernel(int n, double dt, const int* __restrict__ face, const int* __restrict__ axis,
+                             const float* __restrict__ sign, const R* __restrict__ qx, const R* __restrict__ qy,
+                             double* __restrict__ total, double* __restrict__ flow) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= n) return;
+    const R q = axis[m] ? qy[face[m]] : qx[face[m]];
+    const double f = (double)(R(sign[m]) * q);
+    total[m] += dt * f;
+    flow[m] = f;
+}
+
+// Sum of doubles (or of h + res) by one workgroup into out[0]; deterministic order.
+template <class R>
+__global__ __launch_bounds__(1024) void sum_kernel(int64_t n, const int64_t* __restrict__ index,
+                                                   const R* __restrict__ a, const R* __restrict__ b,
+                                                   const double* __restrict__ x, double* __restrict__ out) {
+    __shared__ double red[16];
+    double s = 0.0;
+    for (int64_t p = threadIdx.x; p < n; p += blockDim.x) {
+        if (x)
+            s += x[p];
+        else
+            s += (double)a[index[p]] + (double)b[index[p]];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = 0.0;
+        for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += red[w];
+        out[0] = r;
+    }
+}
+
+// Running maxima of the active tiles' own cells (engine/maxima.py:29-51).
+template <class R>
+__global__ __launch_bounds__(NT) void maxima_kernel(const int* __restrict__ active, int n, const int* __restrict__ nbr,
+                                                    R time, R arrive, R deep, const R* __restrict__ d,
+                                                    const R* __restrict__ h, const R* __restrict__ u,
+                                                    const R* __restrict__ v, R* __restrict__ dmax, R* __restrict__ hmax,
+                                                    R* __restrict__ smax, R* __restrict__ zmax, R* __restrict__ tarr) {
+    if ((int)blockIdx.x >= n) return;
+    const int t = active[blockIdx.x];
+    const int te = nbr[(size_t)t * 8 + 4], ts = nbr[(size_t)t * 8 + 6];
+    for (int p = threadIdx.x; p < TILE2; p += NT) {
+        const int j = p / TILE, i = p - (p / TILE) * TILE;
+        const size_t k = (size_t)t * TILE2 + p;
+        const R dk = d[k];
+        if (dk > dmax[k]) dmax[k] = dk;
+        if (h[k] > hmax[k]) hmax[k] = h[k];
+        if (dk > deep) {
+            const size_t ke = i < TILE - 1 ? k + 1 : (size_t)te * TILE2 + j * TILE;
+            const size_t ks = j < TILE - 1 ? k + TILE : (size_t)ts * TILE2 + i;
+            const R uc = R(0.5) * (u[k] + u[ke]);
+            const R vc = R(0.5) * (v[k] + v[ks]);
+            const R s = sqrt(uc * uc + vc * vc);
+            if (s > smax[k]) smax[k] = s;
+            if (s * dk > zmax[k]) zmax[k] = s * dk;
+        }
+        if (tarr[k] < R(0) && dk >= arrive) tarr[k] = time;
+    }
+}
+
+template <class R>
+__global__ void fill_kernel(R* a, size_t n, R v) {
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) a[p] = v;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------------------------------------------
+// host side
+
+struct w2d_handle {
+    int prec = 4;
+    int fast = 0;
+    int ntiles = 0;
+    int rows = 0;
+    size_t ncell = 0;
+    Params P{};
+    hipStream_t stream = nullptr;
+    // device state, element size prec
+    void* h[2] = {nullptr, nullptr};
+    int hcur = 0;
+    void* uv[4][2] = {};  // u, v, mx, my: [field][buffer]
+    int ucur = 0;
+    void *d = nullptr, *qx = nullptr, *qy = nullptr, *qx1 = nullptr, *qy1 = nullptr, *res = nullptr,
+         *src = nullptr, *om = nullptr;
+    void* wave = nullptr;
+    int* wet = nullptr;
+    // statics
+    float* hc = nullptr;
+    float2* ctab = nullptr;
+    float4 *ftabx = nullptr, *ftaby = nullptr, *sx = nullptr, *sy = nullptr, *ec = nullptr;
+    float *zcx = nullptr, *zcy = nullptr;
+    int* nbr = nullptr;
+    std::vector<int32_t> nbr_host;
+    // lists
+    int* active = nullptr;
+    int nact = 0;
+    std::vector<int32_t> active_host;
+    int* tiles_scratch = nullptr;
+    int nb_cells = 0;
+    int *b_cell = nullptr, *b_ref = nullptr, *b_kind = nullptr;
+    float *b_d0 = nullptr, *b_h0 = nullptr;
+    int n_tally = 0;
+    int *t_face = nullptr, *t_axis = nullptr;
+    float* t_sign = nullptr;
+    double *t_total = nullptr, *t_flow = nullptr;
+    int64_t n_store = 0;
+    int64_t* store_index = nullptr;
+    void* maxima[5] = {};
+    double m_arrive = 0.1, m_deep = 0.05;
+    double* scalar = nullptr;  // reduction results
+    std::vector<void*> owned;
+};
+
+namespace {
+
+size_t esize(const w2d_handle* H) { return H->prec == 8 ? sizeof(double) : sizeof(float); }
+
+int dalloc(w2d_handle* H, void** p, size_t bytes) {
+    hipError_t e = hipMalloc(p, bytes ? bytes : 16);
+    if (e != hipSuccess) return fail(e == hipErrorOutOfMemory ? 2 : 3, std::string("hipMalloc: ") + hipGetErrorString(e));
+    H->owned.push_back(*p);
+    e = hipMemsetAsync(*p, 0, bytes ? bytes : 16, H->stream);
+    if (e != hipSuccess) return fail(3, std::string("hipMemset: ") + hipGetErrorString(e));
+    return 0;
+}
+
+template <class T>
+int upload(w2d_handle* H, T** dst, const T* src, size_t count) {
+    int s = dalloc(H, (void**)dst, count * sizeof(T));
+    if (s) return s;
+    HIP_TRY(hipMemcpyAsync(*dst, src, count * sizeof(T), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+// Device pointer of field f (the current buffer for the swapped ones), or null for a field the device keeps no copy of.
+void* field_ptr(w2d_handle* H, int f) {
+    switch (f) {
+        case W2D_H: return H->h[H->hcur];
+        case W2D_D: return H->d;
+        case W2D_U: return H->uv[0][H->ucur];
+        case W2D_V: return H->uv[1][H->ucur];
+        case W2D_MX: return H->uv[2][H->ucur];
+        case W2D_MY: return H->uv[3][H->ucur];
+        case W2D_QX: return H->qx;
+        case W2D_QY: return H->qy;
+        case W2D_U1: return H->uv[0][1 - H->ucur];
+        case W2D_V1: return H->uv[1][1 - H->ucur];
+        case W2D_MX1: return H->uv[2][1 - H->ucur];
+        case W2D_MY1: return H->uv[3][1 - H->ucur];
+        case W2D_QX1: return H->qx1;
+        case W2D_QY1: return H->qy1;
+        case W2D_SRC: return H->src;
+        case W2D_RES: return H->res;
+        case W2D_OM: return H->om;
+        default: return nullptr;
+    }
+}
+
+template <class R>
+StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
+    StepArgs<R> A{};
+    A.s.h = (R*)H->h[H->hcur];
+    A.s.h_next = (R*)H->h[1 - H->hcur];
+    A.s.d = (R*)H->d;
+    A.s.u = (R*)H->uv[0][H->ucur];
+    A.s.v = (R*)H->uv[1][H->ucur];
+    A.s.mx = (R*)H->uv[2][H->ucur];
+    A.s.my = (R*)H->uv[3][H->ucur];
+    A.s.u1 = (R*)H->uv[0][1 - H->ucur];
+    A.s.v1 = (R*)H->uv[1][1 - H->ucur];
+    A.s.mx1 = (R*)H->uv[2][1 - H->ucur];
+    A.s.my1 = (R*)H->uv[3][1 - H->ucur];
+    A.s.qx = (R*)H->qx;
+    A.s.qy = (R*)H->qy;
+    A.s.qx1 = (R*)H->qx1;
+    A.s.qy1 = (R*)H->qy1;
+    A.s.res = (R*)H->res;
+    A.s.src = (R*)H->src;
+    A.s.om = (R*)H->om;
+    A.st.hc = H->hc;
+    A.st.ctab = H->ctab;
+    A.st.ftabx = H->ftabx;
+    A.st.ftaby = H->ftaby;
+    A.st.sx = H->sx;
+    A.st.sy = H->sy;
+    A.st.ec = H->ec;
+    A.st.zcx = H->zcx;
+    A.st.zcy = H->zcy;
+    A.st.nbr = H->nbr;
+    A.st.ncell = H->ncell;
+    A.st.last = H->rows - 1;
+    A.P = H->P;
+    A.active = H->active;
+    A.nact = H->nact;
+    // the reference hands float32 kernels np.float32(dt) (gpu/backend.py:153)
+    A.dt = (R)dt;
+    A.want_wave = want_wave;
+    A.wave = (R*)H->wave;
+    A.wet = H->wet;
+    return A;
+}
+
+template <class R, bool FAST>
+int launch_step(w2d_handle* H, double dt, int want_wave) {
+    const StepArgs<R> A = step_args<R>(H, dt, want_wave);
+    faces_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
+    HIP_TRY(hipGetLastError());
+    cells_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
+    HIP_TRY(hipGetLastError());
+    if (H->nb_cells) {
+        boundary_kernel<R><<<(H->nb_cells + 255) / 256, 256, 0, H->stream>>>(
+            H->nb_cells, H->b_cell, H->b_ref, H->b_kind, H->b_d0, H->b_h0, (R*)H->h[1 - H->hcur], (R*)H->d);
+        HIP_TRY(hipGetLastError());
+    }
+    return 0;
+}
+
+int run_step(w2d_handle* H, double dt, int want_wave) {
+    if (H->prec == 8) return launch_step<double, false>(H, dt, want_wave);
+    if (H->fast) return launch_step<float, true>(H, dt, want_wave);
+    return launch_step<float, false>(H, dt, want_wave);
+}
+
+// own-cell index (t * 1024 + j * 32 + i) of a state index, or -1 for a halo entry or one out of range
+int64_t own_of(const w2d_handle* H, int64_t k) {
+    if (k < 0 || k >= (int64_t)H->ntiles * STRIDE2) return -1;
+    const int64_t t = k / STRIDE2;
+    const int64_t rem = k - t * STRIDE2;
+    const int jj = (int)(rem / STRIDE), ii = (int)(rem - (rem / STRIDE) * STRIDE);
+    if (jj < HALO || jj >= HALO + TILE || ii < HALO || ii >= HALO + TILE) return -1;
+    return t * TILE2 + (jj - HALO) * TILE + (ii - HALO);
+}
+
+// state layout (with halos) from the own-cell array: every halo entry from its owning tile, the void's zeros
+template <class T>
+void expand(const w2d_handle* H, const T* own, T* out) {
+    const int nt = H->ntiles;
+    for (int t = 0; t < nt; ++t) {
+        for (int jj = 0; jj < STRIDE; ++jj)
+            for (int ii = 0; ii < STRIDE; ++ii) {
+                const int j = jj - HALO, i = ii - HALO;
+                const int dj = j < 0 ? -1 : (j >= TILE ? 1 : 0);
+                const int di = i < 0 ? -1 : (i >= TILE ? 1 : 0);
+                int tt = t;
+                if (dj || di) {
+                    const int q = (dj + 1) * 3 + di + 1;
+                    tt = t == 0 ? 0 : H->nbr_host[(size_t)t * 8 + (q < 4 ? q : q - 1)];
+                }
+                out[(size_t)t * STRIDE2 + jj * STRIDE + ii] =
+                    own[(size_t)tt * TILE2 + (j - dj * TILE) * TILE + (i - di * TILE)];
+            }
+    }
+}
+
+template <class T>
+void compact(const w2d_handle* H, const T* state, T* own) {
+    for (int t = 0; t < H->ntiles; ++t)
+        for (int j = 0; j < TILE; ++j)
+            for (int i = 0; i < TILE; ++i)
+                own[(size_t)t * TILE2 + j * TILE + i] = state[(size_t)t * STRIDE2 + (j + HALO) * STRIDE + i + HALO];
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* w2d_last_error(void) { return g_error.c_str(); }
+
+int w2d_device_count(int32_t* out) {
+    if (!out) return fail(1, "device_count: bad arguments");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *out = n;
+    return 0;
+}
+
+int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int32_t variant, w2d_handle** out) {
+    if (!T || !p || !out || T->ntiles < 1 || T->rows < 2 || (precision != W2D_F32 && precision != W2D_F64))
+        return fail(1, "w2d_create: bad arguments");
+    auto* H = new w2d_handle();
+    *out = nullptr;
+    H->prec = precision;
+    H->fast = precision == W2D_F32 && variant == W2D_FAST;
+    H->ntiles = T->ntiles;
+    H->rows = T->rows;
+    H->ncell = (size_t)T->ntiles * TILE2;
+    H->P = Params{p->g, p->dx, p->e_wet, p->h_cfl, p->u_max, p->lead, p->a_min, T->rows, p->second, p->sheet,
+                  p->mirror, p->carry};
+    int s = 0;
+    auto bail = [&](int code) {
+        w2d_destroy(H);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(3, "hipStreamCreate failed"));
+    const size_t n = H->ncell, es = esize(H);
+    for (int b = 0; b < 2; ++b)
+        if ((s = dalloc(H, &H->h[b], n * es))) return bail(s);
+    for (int f = 0; f < 4; ++f)
+        for (int b = 0; b < 2; ++b)
+            if ((s = dalloc(H, &H->uv[f][b], n * es))) return bail(s);
+    for (void** a : {&H->d, &H->qx, &H->qy, &H->qx1, &H->qy1, &H->res, &H->src, &H->om})
+        if ((s = dalloc(H, a, n * es))) return bail(s);
+    if ((s = dalloc(H, &H->wave, (size_t)H->ntiles * es))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->wet, (size_t)H->ntiles * sizeof(int)))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->active, (size_t)H->ntiles * sizeof(int)))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->tiles_scratch, (size_t)H->ntiles * sizeof(int)))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->scalar, 16 * sizeof(double)))) return bail(s);
+
+    // statics into the device layout
+    const int rows = T->rows, last = rows - 1;
+    {
+        std::vector<float2> ctab((size_t)last * n);
+        for (size_t c = 0; c < n; ++c)
+            for (int r = 0; r < last; ++r) ctab[(size_t)r * n + c] = make_float2(T->dtab[c * rows + r], T->dtab[c * rows + r + 1]);
+        if ((s = upload(H, &H->ctab, ctab.data(), ctab.size()))) return bail(s);
+    }
+    for (int axis = 0; axis < 2; ++axis) {
+        const float* ft = axis ? T->ftaby : T->ftabx;
+        std::vector<float4> f4((size_t)last * n);
+        for (size_t c = 0; c < n; ++c)
+            for (int r = 0; r < last; ++r) {
+                const float* q = ft + (c * rows + r) * 2;
+                f4[(size_t)r * n + c] = make_float4(q[0], q[1], q[2], q[3]);
+            }
+        if ((s = upload(H, axis ? &H->ftaby : &H->ftabx, f4.data(), f4.size()))) return bail(s);
+    }
+    {
+        std::vector<float4> sx(n), sy(n), ec(n);
+        for (size_t c = 0; c < n; ++c) {
+            sx[c] = make_float4(T->flx[c], T->frx[c], T->glx[c], T->grx[c]);
+            sy[c] = make_float4(T->fly[c], T->fry[c], T->gly[c], T->gry[c]);
+            ec[c] = make_float4(T->efx[c], T->cwx[c], T->efy[c], T->cwy[c]);
+        }
+        if ((s = upload(H, &H->sx, sx.data(), n))) return bail(s);
+        if ((s = upload(H, &H->sy, sy.data(), n))) return bail(s);
+        if ((s = upload(H, &H->ec, ec.data(), n))) return bail(s);
+    }
+    if ((s = upload(H, &H->hc, T->hc, n))) return bail(s);
+    H->nbr_host.assign(T->nbr, T->nbr + (size_t)H->ntiles * 8);
+    for (int q = 0; q < 8; ++q) H->nbr_host[q] = 0;  // the void tile's neighbours are the void
+    if ((s = upload(H, &H->nbr, H->nbr_host.data(), H->nbr_host.size()))) return bail(s);
+    {
+        // centre-line beds with their halos filled, as static_cell_state (state.py:216-223)
+        std::vector<float> zx((size_t)H->ntiles * STRIDE2), zy((size_t)H->ntiles * STRIDE2);
+        expand(H, T->zcx, zx.data());
+        expand(H, T->zcy, zy.data());
+        for (size_t q = 0; q < (size_t)STRIDE2; ++q) zx[q] = zy[q] = 0.0f;
+        if ((s = upload(H, &H->zcx, zx.data(), zx.size()))) return bail(s);
+        if ((s = upload(H, &H->zcy, zy.data(), zy.size()))) return bail(s);
+    }
+    std::vector<int32_t> all;
+    for (int t = 1; t < H->ntiles; ++t) all.push_back(t);
+    if ((s = w2d_set_active(H, all.data(), (int)all.size()))) return bail(s);
+    if (hipStreamSynchronize(H->stream) != hipSuccess) return bail(fail(3, "create: device error"));
+    *out = H;
+    return 0;
+}
+
+int w2d_destroy(w2d_handle* H) {
+    if (!H) return 0;
+    if (H->stream) (void)hipStreamSynchronize(H->stream);
+    for (void* p : H->owned) (void)hipFree(p);
+    if (H->stream) (void)hipStreamDestroy(H->stream);
+    delete H;
+    return 0;
+}
+
+int w2d_set_field(w2d_handle* H, int32_t f, const void* host, int64_t n) {
+    if (!H || !host || n != (int64_t)H->ntiles * STRIDE2) return fail(1, "set_field: bad arguments");
+    void* dst = field_ptr(H, f);
+    if (!dst) return fail(1, "set_field: the device keeps no copy of this field");
+    const size_t es = esize(H);
+    std::vector<char> own(H->ncell * es);
+    if (es == 8)
+        compact(H, (const double*)host, (double*)own.data());
+    else
+        compact(H, (const float*)host, (float*)own.data());
+    // the void tile stays zero whatever the host wrote into tile 0
+    std::memset(own.data(), 0, TILE2 * es);
+    HIP_TRY(hipMemcpyAsync(dst, own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
+    if (f == W2D_H) HIP_TRY(hipMemcpyAsync(H->h[1 - H->hcur], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_get_field(w2d_handle* H, int32_t f, void* host, int64_t n) {
+    if (!H || !host || n != (int64_t)H->ntiles * STRIDE2) return fail(1, "get_field: bad arguments");
+    void* src = field_ptr(H, f);
+    if (!src) return fail(1, "get_field: the device keeps no copy of this field (the positivity ratio lives in LDS)");
+    const size_t es = esize(H);
+    std::vector<char> own(H->ncell * es);
+    HIP_TRY(hipMemcpyAsync(own.data(), src, own.size(), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    if (es == 8)
+        expand(H, (const double*)own.data(), (double*)host);
+    else
+        expand(H, (const float*)own.data(), (float*)host);
+    return 0;
+}
+
+int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, void* out) {
+    if (!H || (n && (!index || !out))) return fail(1, "read_cells: bad arguments");
+    void* src = field_ptr(H, f);
+    if (!src) return fail(1, "read_cells: the device keeps no copy of this field");
+    const size_t es = esize(H);
+    for (int64_t m = 0; m < n; ++m) {
+        const int64_t c = own_of(H, index[m]);
+        if (c < 0) return fail(1, "read_cells: an index is not an own cell");
+        HIP_TRY(hipMemcpyAsync((char*)out + m * es, (char*)src + c * es, es, hipMemcpyDeviceToHost, H->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_set_active(w2d_handle* H, const int32_t* tiles, int32_t n) {
+    if (!H || n < 0 || n > H->ntiles || (n && !tiles)) return fail(1, "set_active: bad arguments");
+    std::vector<char> now(H->ntiles, 0);
+    for (int a = 0; a < n; ++a) {
+        if (tiles[a] <= 0 || tiles[a] >= H->ntiles) return fail(1, "set_active: tile out of range");
+        now[tiles[a]] = 1;
+    }
+    // a tile that leaves the list keeps its mean depth in both buffers
+    std::vector<int32_t> leaving;
+    for (int t : H->active_host)
+        if (!now[t]) leaving.push_back(t);
+    if (!leaving.empty()) {
+        HIP_TRY(hipMemcpyAsync(H->tiles_scratch, leaving.data(), leaving.size() * sizeof(int), hipMemcpyHostToDevice, H->stream));
+        if (H->prec == 8)
+            copy_tiles_kernel<double><<<(int)leaving.size(), NT, 0, H->stream>>>(H->tiles_scratch, (int)leaving.size(),
+                                                                                (double*)H->h[H->hcur], (double*)H->h[1 - H->hcur]);
+        else
+            copy_tiles_kernel<float><<<(int)leaving.size(), NT, 0, H->stream>>>(H->tiles_scratch, (int)leaving.size(),
+                                                                               (float*)H->h[H->hcur], (float*)H->h[1 - H->hcur]);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(H->stream));
+    }
+    H->active_host.assign(tiles, tiles + n);
+    H->nact = n;
+    if (n) HIP_TRY(hipMemcpyAsync(H->active, tiles, n * sizeof(int), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_set_boundary(w2d_handle* H, const int32_t* cell, const int32_t* ref, const int32_t* kind, const float* d0,
+                     const float* h0, int32_t n) {
+    if (!H || n < 0 || (n && (!cell || !ref || !kind || !d0 || !h0))) return fail(1, "set_boundary: bad arguments");
+    std::vector<int32_t> c(n), r(n);
+    for (int m = 0; m < n; ++m) {
+        const int64_t a = own_of(H, cell[m]);
+        const int64_t b = own_of(H, ref[m]);
+        if (a < 0 || b < 0) return fail(1, "set_boundary: a boundary or reference cell is not an own cell");
+        c[m] = (int32_t)a;
+        r[m] = (int32_t)b;
+    }
+    int s;
+    H->nb_cells = 0;
+    if (n) {
+        if ((s = upload(H, &H->b_cell, c.data(), n))) return s;
+        if ((s = upload(H, &H->b_ref, r.data(), n))) return s;
+        if ((s = upload(H, &H->b_kind, kind, n))) return s;
+        if ((s = upload(H, &H->b_d0, d0, n))) return s;
+        if ((s = upload(H, &H->b_h0, h0, n))) return s;
+    }
+    H->nb_cells = n;
+    return 0;
+}
+
+int w2d_step(w2d_handle* H, double dt, int32_t want_wave, int32_t want_wet) {
+    (void)want_wet;  // every active tile's flag is written on every step, as the CPU twin does
+    if (!H) return fail(1, "step: no handle");
+    if (want_wave) {
+        if (H->prec == 8)
+            fill_kernel<double><<<64, 256, 0, H->stream>>>((double*)H->wave, (size_t)H->ntiles, 0.0);
+        else
+            fill_kernel<float><<<64, 256, 0, H->stream>>>((float*)H->wave, (size_t)H->ntiles, 0.0f);
+        HIP_TRY(hipGetLastError());
+    }
+    if (H->nact == 0) return 0;
+    int s = run_step(H, dt, want_wave);
+    if (s) return s;
+    H->hcur = 1 - H->hcur;
+    H->ucur = 1 - H->ucur;
+    return 0;
+}
+
+int w2d_max_wave(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "max_wave: bad arguments");
+    if (H->nact == 0) {
+        *out = 0.0;
+        return 0;
+    }
+    if (H->prec == 8)
+        wave_max_kernel<double><<<1, 1024, 0, H->stream>>>(H->active, H->nact, (double*)H->wave, H->scalar);
+    else
+        wave_max_kernel<float><<<1, 1024, 0, H->stream>>>(H->active, H->nact, (float*)H->wave, H->scalar);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, H->scalar, sizeof(double), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_wave(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "wave: bad arguments");
+    if (H->prec == 8) {
+        HIP_TRY(hipMemcpyAsync(out, H->wave, H->ntiles * sizeof(double), hipMemcpyDeviceToHost, H->stream));
+        HIP_TRY(hipStreamSynchronize(H->stream));
+    } else {
+        std::vector<float> w(H->ntiles);
+        HIP_TRY(hipMemcpyAsync(w.data(), H->wave, H->ntiles * sizeof(float), hipMemcpyDeviceToHost, H->stream));
+        HIP_TRY(hipStreamSynchronize(H->stream));
+        for (int t = 0; t < H->ntiles; ++t) out[t] = w[t];
+    }
+    return 0;
+}
+
+int w2d_wet_flags(w2d_handle* H, int32_t* out) {
+    if (!H || !out) return fail(1, "wet_flags: bad arguments");
+    HIP_TRY(hipMemcpyAsync(out, H->wet, H->ntiles * sizeof(int), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_set_wet(w2d_handle* H, const int32_t* flags) {
+    if (!H || !flags) return fail(1, "set_wet: bad arguments");
+    HIP_TRY(hipMemcpyAsync(H->wet, flags, H->ntiles * sizeof(int), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_refresh_plan(w2d_handle* H, const int32_t* tiles, int32_t n) {
+    if (!H || n < 0 || n > H->ntiles || (n && !tiles)) return fail(1, "refresh_plan: bad arguments");
+    if (!n) return 0;
+    for (int a = 0; a < n; ++a)
+        if (tiles[a] <= 0 || tiles[a] >= H->ntiles) return fail(1, "refresh_plan: tile out of range");
+    HIP_TRY(hipMemcpyAsync(H->tiles_scratch, tiles, n * sizeof(int), hipMemcpyHostToDevice, H->stream));
+    const int last = H->rows - 1;
+    if (H->prec == 8)
+        plan_kernel<double, false><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (double*)H->h[H->hcur], (double*)H->om, H->hc,
+                                                           H->ctab, H->ncell, last);
+    else if (H->fast)
+        plan_kernel<float, true><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (float*)H->h[H->hcur], (float*)H->om, H->hc,
+                                                         H->ctab, H->ncell, last);
+    else
+        plan_kernel<float, false><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (float*)H->h[H->hcur], (float*)H->om, H->hc,
+                                                          H->ctab, H->ncell, last);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_refresh_halos(w2d_handle* H) { return H ? 0 : fail(1, "refresh_halos: no handle"); }
+
+int w2d_tally_set(w2d_handle* H, const int32_t* face, const int32_t* axis, const float* sign, int32_t n) {
+    if (!H || n < 0 || (n && (!face || !axis || !sign))) return fail(1, "tally_set: bad arguments");
+    std::vector<int32_t> f(n);
+    for (int m = 0; m < n; ++m) {
+        const int64_t c = own_of(H, face[m]);
+        if (c < 0) return fail(1, "tally_set: a face is not an own face");
+        f[m] = (int32_t)c;
+    }
+    int s;
+    H->n_tally = 0;
+    if (n) {
+        if ((s = upload(H, &H->t_face, f.data(), n))) return s;
+        if ((s = upload(H, &H->t_axis, axis, n))) return s;
+        if ((s = upload(H, &H->t_sign, sign, n))) return s;
+        if ((s = dalloc(H, (void**)&H->t_total, n * sizeof(double)))) return s;
+        if ((s = dalloc(H, (void**)&H->t_flow, n * sizeof(double)))) return s;
+        HIP_TRY(hipStreamSynchronize(H->stream));
+    }
+    H->n_tally = n;
+    return 0;
+}
+
+int w2d_tally_add(w2d_handle* H, double dt) {
+    if (!H) return fail(1, "tally_add: no handle");
+    if (!H->n_tally) return 0;
+    const int n = H->n_tally;
+    if (H->prec == 8)
+        tally_kernel<double><<<(n + 255) / 256, 256, 0, H->stream>>>(n, dt, H->t_face, H->t_axis, H->t_sign,
+                                                                     (double*)H->qx, (double*)H->qy, H->t_total, H->t_flow);
+    else
+        tally_kernel<float><<<(n + 255) / 256, 256, 0, H->stream>>>(n, dt, H->t_face, H->t_axis, H->t_sign,
+                                                                    (float*)H->qx, (float*)H->qy, H->t_total, H->t_flow);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+static int sum_doubles(w2d_handle* H, const double* x, int64_t n, double* out) {
+    sum_kernel<float><<<1, 1024, 0, H->stream>>>(n, nullptr, nullptr, nullptr, x, H->scalar);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, H->scalar, sizeof(double), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_tally_volume(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "tally_volume: bad arguments");
+    if (!H->n_tally) {
+        *out = 0.0;
+        return 0;
+    }
+    return sum_doubles(H, H->t_total, H->n_tally, out);
+}
+
+int w2d_tally_flow(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "tally_flow: bad arguments");
+    if (!H->n_tally) {
+        *out = 0.0;
+        return 0;
+    }
+    return sum_doubles(H, H->t_flow, H->n_tally, out);
+}
+
+int w2d_storage_set(w2d_handle* H, const int64_t* index, int64_t n) {
+    if (!H || n < 0 || (n && !index)) return fail(1, "storage_set: bad arguments");
+    std::vector<int64_t> own(n);
+    for (int64_t m = 0; m < n; ++m) {
+        own[m] = own_of(H, index[m]);
+        if (own[m] < 0) return fail(1, "storage_set: an index is not an own cell");
+    }
+    int s;
+    H->n_store = 0;
+    if (n && (s = upload(H, &H->store_index, own.data(), n))) return s;
+    H->n_store = n;
+    return 0;
+}
+
+int w2d_storage(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "storage: bad arguments");
+    if (!H->n_store) {
+        *out = 0.0;
+        return 0;
+    }
+    if (H->prec == 8)
+        sum_kernel<double><<<1, 1024, 0, H->stream>>>(H->n_store, H->store_index, (double*)H->h[H->hcur],
+                                                      (double*)H->res, nullptr, H->scalar);
+    else
+        sum_kernel<float><<<1, 1024, 0, H->stream>>>(H->n_store, H->store_index, (float*)H->h[H->hcur],
+                                                     (float*)H->res, nullptr, H->scalar);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, H->scalar, sizeof(double), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_maxima_set(w2d_handle* H, double arrive, double deep) {
+    if (!H) return fail(1, "maxima_set: no handle");
+    H->m_arrive = arrive;
+    H->m_deep = deep;
+    const size_t es = esize(H);
+    int s;
+    for (int q = 0; q < 5; ++q) {
+        if (!H->maxima[q] && (s = dalloc(H, &H->maxima[q], H->ncell * es))) return s;
+        if (H->prec == 8)
+            fill_kernel<double><<<256, 256, 0, H->stream>>>((double*)H->maxima[q], H->ncell, q == 4 ? -1.0 : 0.0);
+        else
+            fill_kernel<float><<<256, 256, 0, H->stream>>>((float*)H->maxima[q], H->ncell, q == 4 ? -1.0f : 0.0f);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_maxima_update(w2d_handle* H, double t) {
+    if (!H || !H->maxima[0]) return fail(1, "maxima_update: call maxima_set first");
+    if (!H->nact) return 0;
+    if (H->prec == 8)
+        maxima_kernel<double><<<H->nact, NT, 0, H->stream>>>(
+            H->active, H->nact, H->nbr, t, H->m_arrive, H->m_deep, (double*)H->d, (double*)H->h[H->hcur],
+            (double*)H->uv[0][H->ucur], (double*)H->uv[1][H->ucur], (double*)H->maxima[0], (double*)H->maxima[1],
+            (double*)H->maxima[2], (double*)H->maxima[3], (double*)H->maxima[4]);
+    else
+        maxima_kernel<float><<<H->nact, NT, 0, H->stream>>>(
+            H->active, H->nact, H->nbr, (float)t, (float)H->m_arrive, (float)H->m_deep, (float*)H->d, (float*)H->h[H->hcur],
+            (float*)H->uv[0][H->ucur], (float*)H->uv[1][H->ucur], (float*)H->maxima[0], (float*)H->maxima[1],
+            (float*)H->maxima[2], (float*)H->maxima[3], (float*)H->maxima[4]);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int w2d_maxima_get(w2d_handle* H, int32_t which, void* host, int64_t n) {
+    if (!H || which < 0 || which > 4 || !H->maxima[which] || n != (int64_t)H->ntiles * STRIDE2)
+        return fail(1, "maxima_get: bad arguments");
+    const size_t es = esize(H);
+    std::vector<char> own(H->ncell * es);
+    HIP_TRY(hipMemcpyAsync(own.data(), H->maxima[which], own.size(), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    if (es == 8)
+        expand(H, (const double*)own.data(), (double*)host);
+    else
+        expand(H, (const float*)own.data(), (float*)host);
+    return 0;
+}
+
+int w2d_synchronize(w2d_handle* H) {
+    if (!H) return fail(1, "synchronize: no handle");
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+}  // extern "C"

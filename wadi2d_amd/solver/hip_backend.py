@@ -1,0 +1,246 @@
+"""HipBackend: the reference's backend protocol over the HIP step library (include/wadi2d_hip.h).
+
+It answers what ``CpuBackend`` and ``GpuBackend`` answer (wadi2d/solver/cpu/backend.py:14-141,
+wadi2d/solver/gpu/backend.py:53-203): ``set_active``, ``set_field``/``get_field`` in the state layout,
+``set_boundary``, ``step``, ``max_wave``, ``wet_flags``, ``refresh_plan``, ``refresh_halos``, ``synchronize``; the
+device holds every array. Errors from the library raise: an invalid argument ``ValueError``, out of memory
+``MemoryError``, a device error ``RuntimeError``.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from wadi2d_amd import _native
+from wadi2d_amd.layout import STATE_ARRAYS, STRIDE2, WORK_ARRAYS
+
+FIELDS = STATE_ARRAYS + WORK_ARRAYS
+F32, F64 = 4, 8
+EXACT, FAST = 0, 1
+VP, I32, I64, D = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+
+
+class _Tables(ctypes.Structure):
+    _fields_ = [("ntiles", I32), ("rows", I32), ("nbr", VP)] + [
+        (n, VP)
+        for n in ("hc", "dtab", "flx", "frx", "glx", "grx", "efx", "cwx", "ftabx",
+                  "fly", "fry", "gly", "gry", "efy", "cwy", "ftaby", "zcx", "zcy")  # fmt: skip
+    ]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [(n, D) for n in ("g", "dx", "e_wet", "h_cfl", "u_max", "lead", "a_min")] + [
+        (n, I32) for n in ("second", "sheet", "mirror", "carry")
+    ]
+
+
+_SIGS = {
+    "w2d_create": [ctypes.POINTER(_Tables), ctypes.POINTER(_Params), I32, I32, ctypes.POINTER(VP)],
+    "w2d_destroy": [VP],
+    "w2d_set_field": [VP, I32, VP, I64],
+    "w2d_get_field": [VP, I32, VP, I64],
+    "w2d_read_cells": [VP, I32, VP, I64, VP],
+    "w2d_set_active": [VP, VP, I32],
+    "w2d_set_boundary": [VP, VP, VP, VP, VP, VP, I32],
+    "w2d_step": [VP, D, I32, I32],
+    "w2d_max_wave": [VP, ctypes.POINTER(D)],
+    "w2d_wave": [VP, VP],
+    "w2d_wet_flags": [VP, VP],
+    "w2d_set_wet": [VP, VP],
+    "w2d_refresh_plan": [VP, VP, I32],
+    "w2d_refresh_halos": [VP],
+    "w2d_tally_set": [VP, VP, VP, VP, I32],
+    "w2d_tally_add": [VP, D],
+    "w2d_tally_volume": [VP, ctypes.POINTER(D)],
+    "w2d_tally_flow": [VP, ctypes.POINTER(D)],
+    "w2d_maxima_set": [VP, D, D],
+    "w2d_maxima_update": [VP, D],
+    "w2d_maxima_get": [VP, I32, VP, I64],
+    "w2d_storage_set": [VP, VP, I64],
+    "w2d_storage": [VP, ctypes.POINTER(D)],
+    "w2d_synchronize": [VP],
+    "w2d_device_count": [ctypes.POINTER(I32)],
+}
+
+
+def api() -> ctypes.CDLL:
+    lib = _native.hip()
+    if not getattr(lib, "_w2d_typed", False):
+        for name, args in _SIGS.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        lib.w2d_last_error.restype = ctypes.c_char_p
+        lib.w2d_last_error.argtypes = []
+        lib._w2d_typed = True
+    return lib
+
+
+def check(status: int) -> None:
+    if status == 0:
+        return
+    msg = api().w2d_last_error().decode()
+    if status == 1:
+        raise ValueError(msg)
+    if status == 2:
+        raise MemoryError(msg)
+    raise RuntimeError(msg)
+
+
+def device_count() -> int:
+    n = I32(0)
+    check(api().w2d_device_count(ctypes.byref(n)))
+    return int(n.value)
+
+
+def _c(a: np.ndarray, dtype) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class HipBackend:
+    """Owns the device copies; ``precision`` ``"f32"`` or ``"f64"``, ``variant`` ``"fast"`` or ``"exact"``."""
+
+    name = "hip"
+
+    def __init__(self, tables, num, precision: str = "f32", variant: str = "fast") -> None:
+        self.grid = tables.grid
+        self.num = num
+        self.rows = tables.rows
+        self.precision = precision
+        self.dtype = np.float64 if precision == "f64" else np.float32
+        self.n = self.grid.ntiles * STRIDE2
+        nbr = _c(self.grid.nbr, np.int32)
+        keep = [nbr]
+
+        def f32(a):
+            arr = _c(a, np.float32)
+            keep.append(arr)
+            return arr.ctypes.data
+
+        t = _Tables(self.grid.ntiles, tables.rows, nbr.ctypes.data,
+                    *[f32(getattr(tables, n)) for n in ("hc", "dtab", "flx", "frx", "glx", "grx", "efx", "cwx",
+                      "ftabx", "fly", "fry", "gly", "gry", "efy", "cwy", "ftaby", "zcx", "zcy")])  # fmt: skip
+        p = _Params(num.g, num.dx, num.e_wet, num.h_cfl, num.u_max, num.lead, num.a_min, int(num.second),
+                    int(num.sheet), int(num.mirror), int(num.carry))  # fmt: skip
+        handle = VP()
+        check(api().w2d_create(ctypes.byref(t), ctypes.byref(p), F64 if precision == "f64" else F32,
+                               FAST if variant == "fast" else EXACT, ctypes.byref(handle)))  # fmt: skip
+        self.h = handle
+        self.extra_wave = 0.0
+        self.hooks = None
+        self.active_tiles = np.arange(1, self.grid.ntiles, dtype=np.int32)
+        self.nact = int(self.active_tiles.size)
+        self._tally_n = 0
+
+    def __del__(self) -> None:
+        h = getattr(self, "h", None)
+        if h is not None and h.value:
+            api().w2d_destroy(h)
+            self.h = VP()
+
+    # ---- the protocol
+    def set_active(self, active: np.ndarray) -> None:
+        a = _c(active, np.int32)
+        check(api().w2d_set_active(self.h, a.ctypes.data, int(a.size)))
+        self.active_tiles = a
+        self.nact = int(a.size)
+
+    @property
+    def active(self) -> np.ndarray:
+        return self.active_tiles
+
+    def set_field(self, name: str, a: np.ndarray) -> None:
+        v = _c(a, self.dtype)
+        if v.size != self.n:
+            raise ValueError(f"{name}: {v.size} entries for {self.n}")
+        check(api().w2d_set_field(self.h, FIELDS.index(name), v.ctypes.data, self.n))
+
+    def get_field(self, name: str) -> np.ndarray:
+        out = np.zeros(self.n, self.dtype)
+        check(api().w2d_get_field(self.h, FIELDS.index(name), out.ctypes.data, self.n))
+        return out
+
+    def read_cells(self, name: str, index: np.ndarray) -> np.ndarray:
+        idx = _c(index, np.int64)
+        out = np.zeros(idx.size, self.dtype)
+        check(api().w2d_read_cells(self.h, FIELDS.index(name), idx.ctypes.data, int(idx.size), out.ctypes.data))
+        return out
+
+    def set_boundary(self, bc) -> None:
+        arrs = [_c(bc.cell, np.int32), _c(bc.ref, np.int32), _c(bc.kind, np.int32), _c(bc.d0, np.float32),
+                _c(bc.h0, np.float32)]  # fmt: skip
+        self.bc = bc
+        check(api().w2d_set_boundary(self.h, *[a.ctypes.data for a in arrs], int(arrs[0].size)))
+
+    def set_wet(self, flags: np.ndarray) -> None:
+        f = _c(flags, np.int32)
+        check(api().w2d_set_wet(self.h, f.ctypes.data))
+
+    def refresh_plan(self, tiles: np.ndarray) -> None:
+        t = _c(tiles, np.int32)
+        check(api().w2d_refresh_plan(self.h, t.ctypes.data, int(t.size)))
+
+    def refresh_halos(self) -> None:
+        check(api().w2d_refresh_halos(self.h))
+
+    def step(self, dt: float, want_wave: bool, want_wet: bool) -> None:
+        check(api().w2d_step(self.h, float(dt), int(want_wave), int(want_wet)))
+
+    def max_wave(self) -> float:
+        out = D(0.0)
+        check(api().w2d_max_wave(self.h, ctypes.byref(out)))
+        return max(float(out.value), float(self.extra_wave))
+
+    def wave(self) -> np.ndarray:
+        out = np.zeros(self.grid.ntiles, np.float64)
+        check(api().w2d_wave(self.h, out.ctypes.data))
+        return out
+
+    def wet_flags(self) -> np.ndarray:
+        out = np.zeros(self.grid.ntiles, np.int32)
+        check(api().w2d_wet_flags(self.h, out.ctypes.data))
+        return out
+
+    def synchronize(self) -> None:
+        check(api().w2d_synchronize(self.h))
+
+    # ---- the run's accumulators on the device (engine/maxima.py, engine/monitor.py)
+    def tally_set(self, faces: np.ndarray, axis: np.ndarray, sign: np.ndarray) -> None:
+        f, a, s = _c(faces, np.int32), _c(axis, np.int32), _c(sign, np.float32)
+        check(api().w2d_tally_set(self.h, f.ctypes.data, a.ctypes.data, s.ctypes.data, int(f.size)))
+        self._tally_n = int(f.size)
+
+    def tally_add(self, dt: float) -> None:
+        check(api().w2d_tally_add(self.h, float(dt)))
+
+    def tally_volume(self) -> float:
+        out = D(0.0)
+        check(api().w2d_tally_volume(self.h, ctypes.byref(out)))
+        return float(out.value)
+
+    def tally_flow(self) -> float:
+        out = D(0.0)
+        check(api().w2d_tally_flow(self.h, ctypes.byref(out)))
+        return float(out.value)
+
+    def maxima_set(self, arrival_depth: float, speed_depth: float) -> None:
+        check(api().w2d_maxima_set(self.h, float(arrival_depth), float(speed_depth)))
+
+    def maxima_update(self, t: float) -> None:
+        check(api().w2d_maxima_update(self.h, float(t)))
+
+    def maxima_get(self, which: int) -> np.ndarray:
+        out = np.zeros(self.n, self.dtype)
+        check(api().w2d_maxima_get(self.h, int(which), out.ctypes.data, self.n))
+        return out
+
+    def storage_set(self, index: np.ndarray) -> None:
+        idx = _c(index, np.int64)
+        check(api().w2d_storage_set(self.h, idx.ctypes.data, int(idx.size)))
+
+    def storage_sum(self) -> float:
+        out = D(0.0)
+        check(api().w2d_storage(self.h, ctypes.byref(out)))
+        return float(out.value)
