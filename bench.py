@@ -1,0 +1,183 @@
+"""Benchmark of the uniform-grid step on one MI355X: cell-updates/s and the fraction of HBM peak.
+
+Workload: the full-extent plain FP32 step, 4096 x 4096 cells at 30 m (16,777,216 cells), seeded synthetic ground,
+12-row sub-grid tables, a Gaussian storm cell on a triangular 3 h hyetograph, four free outflow segments
+(``wadi2d_amd.synth.full_extent_case``). The run is taken hot to t = 1.5 h (the storm's peak) through the same
+window loop a run uses, then W steps of warm-up and exactly K timed steps. A step is what the stepper does per
+step: both kernels, the boundary cells, the FP64 boundary tally, a wave-speed reduction every 8 steps, the
+active-tile check every 16 steps and a source refresh every 30 s of simulated time.
+
+Printed: one JSON line. ``value`` counts domain cells (padding excluded) times steps over the wall time of the
+timed region, with the inputs resident in HBM. ``roofline`` is the face kernel's (the dominant one): its
+algorithmic bytes per launch over its mean launch time from HIP events on the library's stream. ``cpu_baseline``
+is the oracle's float instantiation (the reference's float32 statement order, OpenMP over tiles) on the host's
+cores, timed on a bounded number of steps from the same hot state.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
+# algorithmic bytes per own cell (float32), each array counted once per cell per step
+FACE_BYTES = 144  # h,d,u,v,mx,my,qx,qy 32 + zcx,zcy 8 + face statics 48 + two table pair reads 32 + 6 writes 24
+CELL_BYTES = 52   # h,qx1,qy1,res,src 20 + Hc 4 + level pair 8 + writes qx,qy,h,d,res 20
+STEP_BYTES = 176  # the whole step as one pass: 140 read + 36 written
+
+
+def log(*a) -> None:
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--size", type=int, default=4096)
+    ap.add_argument("--hot", type=float, default=5400.0, help="simulated seconds before the timed steps")
+    ap.add_argument("--variant", choices=("fast", "exact"), default="fast")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample length; 0 skips it")
+    ap.add_argument("--events", choices=("on", "off"), default="on", help="HIP events around the kernels")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
+        return 2
+
+    from wadi2d_amd import synth
+    from wadi2d_amd.engine.driver import Driver
+    from wadi2d_amd.engine.forcing import Rain, boundary_set
+    from wadi2d_amd.solver.model import Model
+
+    t0 = time.time()
+    scen = synth.full_extent_case(args.size)
+    log(f"[bench] ground {scen.name}: {time.time() - t0:.1f} s")
+    t0 = time.time()
+    tables = scen.build()
+    log(f"[bench] tables: {time.time() - t0:.1f} s, {tables.grid.ntiles - 1} tiles")
+    bset = boundary_set(synth.edge_outlets(scen), tables)
+    rain = Rain(tables, bset, scen.cumulative_mm, scen.weight)
+    del scen.bed
+    backend = "hip" if args.variant == "fast" else "hip-exact"
+    t0 = time.time()
+    model = Model(tables, scen.numerics, backend)
+    model.set_depth(np.zeros(tables.grid.shape))
+    drv = Driver(model, bset, rain)
+    log(f"[bench] device upload: {time.time() - t0:.1f} s")
+    b = model.backend
+    st = model.stepper
+    t0 = time.time()
+    hot = args.hot
+    while st.t < hot - 1e-9:
+        drv.run(min(st.t + 600.0, hot))
+        log(f"[bench] hot start t={st.t:.0f} s steps={st.nsteps} dt={st.dt:.3f} active={b.nact}")
+    log(f"[bench] hot start: {time.time() - t0:.1f} s")
+    drv.run(1e12, max_steps=st.nsteps + args.warmup)
+    b.synchronize()
+    b.timing_read()
+    b.timing(args.events == "on")
+    n0, tsim0 = st.nsteps, st.t
+    t0 = time.perf_counter()
+    drv.run(1e12, max_steps=st.nsteps + args.steps)
+    b.synchronize()
+    wall = time.perf_counter() - t0
+    b.timing(False)
+    faces_ms, cells_ms, launches = b.timing_read()
+    steps = st.nsteps - n0
+    assert steps == args.steps and launches in (0, steps), (steps, launches)
+    launches = max(launches, 1)
+    domain_cells = int(tables.grid.mask.sum())
+    tiles = int(b.nact)
+    value = domain_cells * steps / wall
+    face_s = faces_ms / 1e3 / launches
+    cell_s = cells_ms / 1e3 / launches
+    face_achieved = FACE_BYTES * tiles * 1024 / face_s if face_s > 0 else 0.0
+    bal = drv.accounts()
+    log(f"[bench] timed {steps} steps in {wall:.3f} s ({wall / steps * 1e3:.3f} ms/step), simulated "
+        f"{st.t - tsim0:.0f} s; faces {face_s * 1e3:.3f} ms, cells {cell_s * 1e3:.3f} ms per launch; "
+        f"active tiles {tiles}; mass error {bal.relative:.2e}")
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(model, tables, scen.numerics, args.cpu_seconds)
+
+    out = {
+        "metric": "cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded ground, tables and storm built in-process)",
+        "config": {
+            "workload": f"plain FP32 step, {args.size}x{args.size} cells at 30 m, 12-row sub-grid tables, hot start "
+                        f"at t={hot:.0f} s of a Gaussian 80 mm/h storm, four free outflows",
+            "cells": domain_cells,
+            "active_tiles": tiles,
+            "variant": args.variant,
+            "hbm_frac_step": STEP_BYTES * value / HBM_PEAK,
+            "faces_ms": face_s * 1e3,
+            "cells_ms": cell_s * 1e3,
+            "mass_error": bal.relative,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "faces_kernel",
+            "achieved": face_achieved / 1e9,
+            "peak": HBM_PEAK / 1e9,
+            "unit": "GB/s",
+            "frac": face_achieved / HBM_PEAK,
+            "traffic": None,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def cpu_baseline(model, tables, num, seconds: float):
+    """The oracle's float32 step on the host's cores, from the GPU's hot state, for about ``seconds``."""
+    from oracle.oracle import Numerics as ONumerics
+    from oracle.oracle import OracleBackend
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    t0 = time.time()
+    ob = OracleBackend(tables, ONumerics.like(num), prec=4, threads=cores)
+    for f in ("h", "d", "u", "v", "mx", "my", "qx", "qy", "res", "om", "src"):
+        ob.set_field(f, model.backend.get_field(f))
+    ob.set_boundary(model.backend.bc)
+    log(f"[bench] cpu baseline set up in {time.time() - t0:.1f} s on {cores} threads")
+    dt = float(model.stepper.dt)
+    cells = int(tables.grid.mask.sum())
+    done, spent = 0, 0.0
+    while spent < seconds:
+        t = time.perf_counter()
+        ob.step(dt, done % 8 == 0, done % 16 == 0)
+        spent += time.perf_counter() - t
+        done += 1
+        log(f"[bench] cpu step {done}: {spent:.1f} s")
+    return {
+        "value": cells * done / spent,
+        "unit": "cell-updates/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"{done} steps of the full {cells}-cell grid from the same hot state (float32, OpenMP over tiles)",
+    }
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -94,6 +94,16 @@ int w2d_maxima_get(w2d_handle* h, int32_t which, void* host, int64_t n);
 /* Monitor.storage (engine/monitor.py:214-218) without the dx^2: sum of h + res over listed own cells, FP64. */
 int w2d_storage_set(w2d_handle* h, const int64_t* index, int64_t n);
 int w2d_storage(w2d_handle* h, double* out);
+/* A source that is a fixed weight per cell times one number per window (no counterpart: the reference uploads the
+ * window's float32 rates, model.py:112-122): weights at own-cell state indices, then per window
+ * src = float(weight * scale) on the device, and the sum of src over all cells. */
+int w2d_source_weights(w2d_handle* h, const int64_t* index, const double* weight, int64_t n);
+int w2d_source_scale(w2d_handle* h, double scale);
+int w2d_source_sum(w2d_handle* h, double* out);
+/* Kernel timing (no counterpart): with timing on, each step records HIP events on the handle's stream around its
+ * face and cell kernels; timing_read waits, and returns and clears the summed milliseconds and the step count. */
+int w2d_timing(w2d_handle* h, int32_t on);
+int w2d_timing_read(w2d_handle* h, double* faces_ms, double* cells_ms, int64_t* launches);
 /* synchronize (gpu/backend.py:202-203). */
 int w2d_synchronize(w2d_handle* h);
 const char* w2d_last_error(void);

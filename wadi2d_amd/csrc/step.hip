@@ -609,10 +609,12 @@ __global__ __launch_bounds__(1024) void sum_kernel(int64_t n, const int64_t* __r
     __shared__ double red[16];
     double s = 0.0;
     for (int64_t p = threadIdx.x; p < n; p += blockDim.x) {
-        if (x)
+        if (x) {
             s += x[p];
-        else
-            s += (double)a[index[p]] + (double)b[index[p]];
+        } else {
+            const int64_t q = index ? index[p] : p;
+            s += (double)a[q] + (b ? (double)b[q] : 0.0);
+        }
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
@@ -622,6 +624,25 @@ __global__ __launch_bounds__(1024) void sum_kernel(int64_t n, const int64_t* __r
         for (int w = 0; w < (int)(blockDim.x >> 6); ++w) r += red[w];
         out[0] = r;
     }
+}
+
+// First stage of a deterministic sum: block b sums entries b, b + B, b + 2B, ... of (a[q] + b[q]) over q = index[p]
+// (or p), in a fixed order; `partial` gets one double per block. The second stage is sum_kernel over `partial`.
+constexpr int SUM_BLOCKS = 1024;
+template <class R>
+__global__ __launch_bounds__(256) void partial_sum_kernel(int64_t n, const int64_t* __restrict__ index,
+                                                          const R* __restrict__ a, const R* __restrict__ b,
+                                                          double* __restrict__ partial) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = index ? index[p] : p;
+        s += (double)a[q] + (b ? (double)b[q] : 0.0);
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
 }
 
 // Running maxima of the active tiles' own cells (engine/maxima.py:29-51).
@@ -651,6 +672,13 @@ __global__ __launch_bounds__(NT) void maxima_kernel(const int* __restrict__ acti
         }
         if (tarr[k] < R(0) && dk >= arrive) tarr[k] = time;
     }
+}
+
+// The window's source rate of every cell from its weight: float(weight * scale), the product in double.
+template <class R>
+__global__ void scale_source_kernel(size_t n, const double* __restrict__ w, double s, R* __restrict__ src) {
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
+        src[p] = (R)(w[p] * s);
 }
 
 template <class R>
@@ -704,6 +732,12 @@ struct w2d_handle {
     void* maxima[5] = {};
     double m_arrive = 0.1, m_deep = 0.05;
     double* scalar = nullptr;  // reduction results
+    double* src_weight = nullptr;
+    double* partial = nullptr;  // SUM_BLOCKS doubles
+    // kernel timing: when on, every step records events around its two kernels on the handle's stream
+    int timing = 0;
+    std::vector<hipEvent_t> event_pool;
+    std::vector<hipEvent_t> marks;  // triples: before faces, after faces, after cells
     std::vector<void*> owned;
 };
 
@@ -797,13 +831,30 @@ StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     return A;
 }
 
+int mark(w2d_handle* H) {
+    hipEvent_t e;
+    if (H->event_pool.empty()) {
+        HIP_TRY(hipEventCreate(&e));
+    } else {
+        e = H->event_pool.back();
+        H->event_pool.pop_back();
+    }
+    HIP_TRY(hipEventRecord(e, H->stream));
+    H->marks.push_back(e);
+    return 0;
+}
+
 template <class R, bool FAST>
 int launch_step(w2d_handle* H, double dt, int want_wave) {
     const StepArgs<R> A = step_args<R>(H, dt, want_wave);
+    int s;
+    if (H->timing && (s = mark(H))) return s;
     faces_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
     HIP_TRY(hipGetLastError());
+    if (H->timing && (s = mark(H))) return s;
     cells_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
     HIP_TRY(hipGetLastError());
+    if (H->timing && (s = mark(H))) return s;
     if (H->nb_cells) {
         boundary_kernel<R><<<(H->nb_cells + 255) / 256, 256, 0, H->stream>>>(
             H->nb_cells, H->b_cell, H->b_ref, H->b_kind, H->b_d0, H->b_h0, (R*)H->h[1 - H->hcur], (R*)H->d);
@@ -857,6 +908,18 @@ void compact(const w2d_handle* H, const T* state, T* own) {
                 own[(size_t)t * TILE2 + j * TILE + i] = state[(size_t)t * STRIDE2 + (j + HALO) * STRIDE + i + HALO];
 }
 
+// sum of a[q] (+ b[q]) over q in index (or 0 .. n-1), FP64, deterministic: many blocks, then one.
+template <class R>
+int big_sum(w2d_handle* H, int64_t n, const int64_t* index, const R* a, const R* b, double* out) {
+    partial_sum_kernel<R><<<SUM_BLOCKS, 256, 0, H->stream>>>(n, index, a, b, H->partial);
+    HIP_TRY(hipGetLastError());
+    sum_kernel<float><<<1, 1024, 0, H->stream>>>(SUM_BLOCKS, nullptr, nullptr, nullptr, H->partial, H->scalar);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, H->scalar, sizeof(double), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -903,6 +966,7 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     if ((s = dalloc(H, (void**)&H->active, (size_t)H->ntiles * sizeof(int)))) return bail(s);
     if ((s = dalloc(H, (void**)&H->tiles_scratch, (size_t)H->ntiles * sizeof(int)))) return bail(s);
     if ((s = dalloc(H, (void**)&H->scalar, 16 * sizeof(double)))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->partial, SUM_BLOCKS * sizeof(double)))) return bail(s);
 
     // statics into the device layout
     const int rows = T->rows, last = rows - 1;
@@ -957,6 +1021,8 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
 int w2d_destroy(w2d_handle* H) {
     if (!H) return 0;
     if (H->stream) (void)hipStreamSynchronize(H->stream);
+    for (hipEvent_t e : H->marks) (void)hipEventDestroy(e);
+    for (hipEvent_t e : H->event_pool) (void)hipEventDestroy(e);
     for (void* p : H->owned) (void)hipFree(p);
     if (H->stream) (void)hipStreamDestroy(H->stream);
     delete H;
@@ -1012,6 +1078,7 @@ int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, vo
 
 int w2d_set_active(w2d_handle* H, const int32_t* tiles, int32_t n) {
     if (!H || n < 0 || n > H->ntiles || (n && !tiles)) return fail(1, "set_active: bad arguments");
+    if (n == H->nact && std::equal(tiles, tiles + n, H->active_host.begin())) return 0;
     std::vector<char> now(H->ntiles, 0);
     for (int a = 0; a < n; ++a) {
         if (tiles[a] <= 0 || tiles[a] >= H->ntiles) return fail(1, "set_active: tile out of range");
@@ -1231,15 +1298,8 @@ int w2d_storage(w2d_handle* H, double* out) {
         return 0;
     }
     if (H->prec == 8)
-        sum_kernel<double><<<1, 1024, 0, H->stream>>>(H->n_store, H->store_index, (double*)H->h[H->hcur],
-                                                      (double*)H->res, nullptr, H->scalar);
-    else
-        sum_kernel<float><<<1, 1024, 0, H->stream>>>(H->n_store, H->store_index, (float*)H->h[H->hcur],
-                                                     (float*)H->res, nullptr, H->scalar);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(out, H->scalar, sizeof(double), hipMemcpyDeviceToHost, H->stream));
-    HIP_TRY(hipStreamSynchronize(H->stream));
-    return 0;
+        return big_sum<double>(H, H->n_store, H->store_index, (double*)H->h[H->hcur], (double*)H->res, out);
+    return big_sum<float>(H, H->n_store, H->store_index, (float*)H->h[H->hcur], (float*)H->res, out);
 }
 
 int w2d_maxima_set(w2d_handle* H, double arrive, double deep) {
@@ -1288,6 +1348,65 @@ int w2d_maxima_get(w2d_handle* H, int32_t which, void* host, int64_t n) {
         expand(H, (const double*)own.data(), (double*)host);
     else
         expand(H, (const float*)own.data(), (float*)host);
+    return 0;
+}
+
+int w2d_source_weights(w2d_handle* H, const int64_t* index, const double* w, int64_t n) {
+    if (!H || n < 0 || (n && (!index || !w))) return fail(1, "source_weights: bad arguments");
+    std::vector<double> own(H->ncell, 0.0);
+    for (int64_t m = 0; m < n; ++m) {
+        const int64_t c = own_of(H, index[m]);
+        if (c < 0) return fail(1, "source_weights: an index is not an own cell");
+        own[c] = w[m];
+    }
+    if (!H->src_weight) {
+        int s = dalloc(H, (void**)&H->src_weight, H->ncell * sizeof(double));
+        if (s) return s;
+    }
+    HIP_TRY(hipMemcpyAsync(H->src_weight, own.data(), H->ncell * sizeof(double), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_source_scale(w2d_handle* H, double scale) {
+    if (!H || !H->src_weight) return fail(1, "source_scale: call source_weights first");
+    if (H->prec == 8)
+        scale_source_kernel<double><<<1024, 256, 0, H->stream>>>(H->ncell, H->src_weight, scale, (double*)H->src);
+    else
+        scale_source_kernel<float><<<1024, 256, 0, H->stream>>>(H->ncell, H->src_weight, scale, (float*)H->src);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+int w2d_source_sum(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "source_sum: bad arguments");
+    if (H->prec == 8) return big_sum<double>(H, (int64_t)H->ncell, nullptr, (double*)H->src, nullptr, out);
+    return big_sum<float>(H, (int64_t)H->ncell, nullptr, (float*)H->src, nullptr, out);
+}
+
+int w2d_timing(w2d_handle* H, int32_t on) {
+    if (!H) return fail(1, "timing: no handle");
+    H->timing = on != 0;
+    return 0;
+}
+
+int w2d_timing_read(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches) {
+    if (!H || !faces_ms || !cells_ms || !launches) return fail(1, "timing_read: bad arguments");
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    double f = 0.0, c = 0.0;
+    const size_t n = H->marks.size() / 3;
+    for (size_t q = 0; q < n; ++q) {
+        float a = 0.0f, b = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&a, H->marks[3 * q], H->marks[3 * q + 1]));
+        HIP_TRY(hipEventElapsedTime(&b, H->marks[3 * q + 1], H->marks[3 * q + 2]));
+        f += a;
+        c += b;
+    }
+    for (hipEvent_t e : H->marks) H->event_pool.push_back(e);
+    H->marks.clear();
+    *faces_ms = f;
+    *cells_ms = c;
+    *launches = (int64_t)n;
     return 0;
 }
 

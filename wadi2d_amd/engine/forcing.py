@@ -1,0 +1,179 @@
+"""Boundary cells, their open faces, and the rain source of the interior cells (wadi2d/forcing.py).
+
+``boundary_set`` takes boundary cells as (row, column) lists of the padded grid; the reference finds them along
+drawn lines with shapely (forcing.py:248-296), which this path does not need: its scenarios name their cells.
+Kinds, references, prescribed values and the open-face set follow forcing.py:299-381.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from wadi2d_amd.config import MM_PER_M
+from wadi2d_amd.solver.backend import BoundaryCells
+from wadi2d_amd.solver.state import state_index
+from wadi2d_amd.tables import hbar_at_level
+
+KIND_FIXED, KIND_FREE, KIND_FOLLOW = 0, 1, 2
+
+
+@dataclass
+class Boundary:
+    """One boundary: ``kind`` "free", "stage" (level ``stage``) or "normal" (follows its inner cell, ``slope``)."""
+
+    kind: str
+    jj: np.ndarray
+    ii: np.ndarray
+    stage: float = 0.0
+    slope: float = 0.001
+
+
+@dataclass
+class BoundarySet:
+    """Boundary cells, and the faces where water crosses into them (forcing.py:207-222)."""
+
+    cells: BoundaryCells
+    mask: np.ndarray
+    xfaces: np.ndarray
+    xsign: np.ndarray
+    yfaces: np.ndarray
+    ysign: np.ndarray
+
+    @property
+    def size(self) -> int:
+        return int(self.cells.cell.size)
+
+
+def _one_boundary(b: Boundary, tables, taken: np.ndarray):
+    """Kind, follow reference and prescribed values of one boundary's cells (forcing.py:299-332)."""
+    grid = tables.grid
+    j, i = np.asarray(b.jj, np.int64), np.asarray(b.ii, np.int64)
+    n = j.size
+    zmin = tables.zmin[j, i]
+    kind = np.full(n, KIND_FIXED, np.int32)
+    ref = np.array(state_index(grid, j, i), np.int64)
+    d0 = np.zeros(n)
+    h0 = np.zeros(n)
+    if b.kind == "stage":
+        hc = grid.from_tiles(tables.hc)[j, i]
+        dtab = grid.from_tiles(tables.dtab)[j, i]
+        d0 = np.maximum(b.stage - zmin, 0.0)
+        h0 = np.asarray(hbar_at_level(b.stage, zmin, hc, dtab))
+    elif b.kind == "free":
+        kind[:] = KIND_FREE
+    else:
+        kind[:] = KIND_FOLLOW
+        ny, nx = grid.shape
+        for m in range(n):
+            options = [
+                (a, c)
+                for a, c in ((j[m] - 1, i[m]), (j[m] + 1, i[m]), (j[m], i[m] - 1), (j[m], i[m] + 1))
+                if 0 <= a < ny and 0 <= c < nx and grid.mask[a, c] and not taken[a, c]
+            ]
+            if not options:
+                kind[m] = KIND_FREE
+                continue
+            a, c = max(options, key=lambda cell: tables.zbar[cell])
+            ref[m] = state_index(grid, np.array(a), np.array(c))
+            d0[m] = min(tables.zmin[a, c] - zmin[m] - b.slope * grid.dx, 0.0)
+    return kind, ref, d0, h0
+
+
+def open_faces(grid, inner: np.ndarray, taken: np.ndarray):
+    """The x and y faces between ``inner`` and ``taken`` cells, with the sign of a flow into ``taken``
+    (forcing.py:335-356)."""
+    faces = []
+    for axis in (1, 0):
+        out_low = inner & np.roll(taken, -1, axis=axis)
+        j, i = np.nonzero(out_low)
+        if axis == 1:
+            i = i + 1
+        else:
+            j = j + 1
+        into = inner & np.roll(taken, 1, axis=axis)
+        ja, ia = np.nonzero(into)
+        index = np.concatenate((state_index(grid, j, i), state_index(grid, ja, ia))).astype(np.int64)
+        sign = np.concatenate((np.ones(j.size), -np.ones(ja.size)))
+        faces.append((index, sign))
+    return faces
+
+
+def boundary_set(boundaries: list[Boundary], tables) -> BoundarySet:
+    """All boundary cells, and the faces between them and the interior with the sign of an outflow
+    (forcing.py:359-381)."""
+    grid = tables.grid
+    taken = np.zeros(grid.shape, bool)
+    for b in boundaries:
+        taken[np.asarray(b.jj), np.asarray(b.ii)] = True
+    parts = []
+    for b in boundaries:
+        kind, ref, d0, h0 = _one_boundary(b, tables, taken)
+        parts.append((np.array(state_index(grid, np.asarray(b.jj), np.asarray(b.ii)), np.int32),
+                      ref.astype(np.int32), kind, d0, h0))  # fmt: skip
+    if not parts:
+        cells = BoundaryCells.empty()
+    else:
+        cat = [np.concatenate(a) for a in zip(*parts, strict=True)]
+        cells = BoundaryCells(cat[0], cat[1], cat[2].astype(np.int32), cat[3].astype(np.float32),
+                              cat[4].astype(np.float32))  # fmt: skip
+    faces = open_faces(grid, grid.mask & ~taken, taken)
+    return BoundarySet(cells, taken, faces[0][0], faces[0][1], faces[1][0], faces[1][1])
+
+
+class Rain:
+    """A storm as a cumulative depth curve times a per-cell weight (1 for a uniform storm), over the interior cells.
+
+    ``rate(t0, t1)`` is the reference's window mean (forcing.py:667-678): the difference of cumulative depth over
+    the window, in m/s, taken to float32 per cell, and the FP64 volume of the window from those float32 rates.
+    """
+
+    def __init__(self, tables, boundary: BoundarySet, cumulative_mm, weight: np.ndarray | None = None) -> None:
+        grid = tables.grid
+        interior = grid.mask & ~boundary.mask
+        self.jj, self.ii = np.nonzero(interior)
+        self.state = np.asarray(state_index(grid, self.jj, self.ii), np.int64)
+        self.area = grid.dx**2
+        self.cumulative_mm = cumulative_mm
+        self.weight = None if weight is None else np.asarray(weight, np.float64)[self.jj, self.ii]
+
+    def scale(self, t0: float, t1: float) -> float:
+        """The window's mean rate per unit weight, m/s (float64)."""
+        return (float(self.cumulative_mm(t1)) - float(self.cumulative_mm(t0))) / (MM_PER_M * (t1 - t0))
+
+    def rate(self, t0: float, t1: float) -> tuple[np.ndarray, float]:
+        s = self.scale(t0, t1)
+        if self.weight is None:
+            out = np.full(self.state.size, np.float32(s), np.float32)
+        else:
+            out = (self.weight * s).astype(np.float32)
+        rain = float(out.sum(dtype=np.float64)) * self.area * (t1 - t0)
+        return out, rain
+
+
+def uniform_storm(depth_mm: float, duration_s: float):
+    """Cumulative depth of a constant-rate storm, mm (storm/uniform.py:32-35)."""
+
+    def cumulative(t: float) -> float:
+        return depth_mm * min(max(t / duration_s, 0.0), 1.0)
+
+    return cumulative
+
+
+def triangular_storm(peak_mm_h: float, rise_s: float, fall_s: float):
+    """Cumulative depth of a triangular hyetograph, 0 -> peak at ``rise_s`` -> 0 at ``rise_s + fall_s``, mm."""
+    p = peak_mm_h / 3600.0
+
+    def cumulative(t: float) -> float:
+        if t <= 0.0:
+            return 0.0
+        if t <= rise_s:
+            return p * t * t / (2.0 * rise_s)
+        up = p * rise_s / 2.0
+        if t <= rise_s + fall_s:
+            x = t - rise_s
+            return up + p * (x - x * x / (2.0 * fall_s))
+        return up + p * fall_s / 2.0
+
+    return cumulative

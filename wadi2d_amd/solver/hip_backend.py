@@ -61,7 +61,12 @@ _SIGS = {
     "w2d_storage_set": [VP, VP, I64],
     "w2d_storage": [VP, ctypes.POINTER(D)],
     "w2d_synchronize": [VP],
+    "w2d_source_weights": [VP, VP, VP, I64],
+    "w2d_source_scale": [VP, D],
+    "w2d_source_sum": [VP, ctypes.POINTER(D)],
     "w2d_device_count": [ctypes.POINTER(I32)],
+    "w2d_timing": [VP, I32],
+    "w2d_timing_read": [VP, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(I64)],
 }
 
 
@@ -235,6 +240,27 @@ class HipBackend:
         out = np.zeros(self.n, self.dtype)
         check(api().w2d_maxima_get(self.h, int(which), out.ctypes.data, self.n))
         return out
+
+    def source_weights(self, index: np.ndarray, weight: np.ndarray) -> None:
+        idx, w = _c(index, np.int64), _c(weight, np.float64)
+        check(api().w2d_source_weights(self.h, idx.ctypes.data, w.ctypes.data, int(idx.size)))
+
+    def source_scale(self, scale: float) -> None:
+        check(api().w2d_source_scale(self.h, float(scale)))
+
+    def source_sum(self) -> float:
+        out = D(0.0)
+        check(api().w2d_source_sum(self.h, ctypes.byref(out)))
+        return float(out.value)
+
+    def timing(self, on: bool) -> None:
+        check(api().w2d_timing(self.h, int(on)))
+
+    def timing_read(self) -> tuple[float, float, int]:
+        """Summed face-kernel and cell-kernel milliseconds since the last read, and the steps they cover."""
+        f, c, n = D(0.0), D(0.0), I64(0)
+        check(api().w2d_timing_read(self.h, ctypes.byref(f), ctypes.byref(c), ctypes.byref(n)))
+        return float(f.value), float(c.value), int(n.value)
 
     def storage_set(self, index: np.ndarray) -> None:
         idx = _c(index, np.int64)
