@@ -48,6 +48,7 @@ def main() -> int:
     ap.add_argument("--variant", choices=("fast", "exact"), default="fast")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample length; 0 skips it")
     ap.add_argument("--events", choices=("on", "off"), default="on", help="HIP events around the kernels")
+    ap.add_argument("--copy-probe", type=float, default=0.0, help="MiB of a plain device copy to time first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -55,6 +56,12 @@ def main() -> int:
 
     from wadi2d_amd import synth
     from wadi2d_amd.engine.driver import Driver
+    from wadi2d_amd.solver.hip_backend import copy_probe
+
+    if args.copy_probe > 0:
+        nbytes = int(args.copy_probe * 2**20)
+        ms = copy_probe(nbytes, 10)
+        log(f"[bench] copy probe: {nbytes} B read + {nbytes} B written in {ms:.4f} ms = {2 * nbytes / ms / 1e9:.0f} GB/s")
     from wadi2d_amd.engine.forcing import Rain, boundary_set
     from wadi2d_amd.solver.model import Model
 

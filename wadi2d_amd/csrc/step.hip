@@ -99,6 +99,19 @@ __device__ __forceinline__ R pmax(R a, R b) { return b > a ? b : a; }
 template <class R>
 __device__ __forceinline__ R pmin(R a, R b) { return b < a ? b : a; }
 
+// max/min of the step: Python's order in the exact variants, the hardware's v_max/v_min (which differ only for a
+// NaN and the sign of a zero) in the fast one.
+template <bool FAST, class R>
+__device__ __forceinline__ R fmx(R a, R b) {
+    if constexpr (FAST) return fmax(a, b);
+    return pmax(a, b);
+}
+template <bool FAST, class R>
+__device__ __forceinline__ R fmn(R a, R b) {
+    if constexpr (FAST) return fmin(a, b);
+    return pmin(a, b);
+}
+
 // int(s) for the row of a table, capped; NaN and negative give row 0.
 template <class R>
 __device__ __forceinline__ int row_of(R s, int cap) {
@@ -140,6 +153,25 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
         W = dv<FAST>(a1 - a0, E * (R(1) - c * c));
         A = a1 + W * (e - E);
         T = R(q.w) * dv<FAST>(A, pmax(a1, R(1e-12)));
+        return;
+    }
+    if constexpr (FAST) {
+        // the same rows, fewer divisions: depth over the top read once as a ratio, one reciprocal per row gap
+        const R x = e * __builtin_amdgcn_rcpf(E);
+        const R s = sq<FAST>(x) * m;
+        const int r = row_of(s, last - 1);
+        const float4 q = tab[(size_t)r * plane];
+        const R im = __builtin_amdgcn_rcpf(m);
+        const R c0 = R(r) * im;
+        const R c1 = R(r + 1) * im;
+        const R c0s = c0 * c0;
+        const R gap = c1 * c1 - c0s;
+        const R ig = __builtin_amdgcn_rcpf(gap);
+        const R f = (x - c0s) * ig;
+        const R a0 = R(q.x), a1 = R(q.z), t0 = R(q.y);
+        A = a0 + f * (a1 - a0);
+        T = t0 + f * (R(q.w) - t0);
+        W = (a1 - a0) * ig * __builtin_amdgcn_rcpf(E);
         return;
     }
     const R s = sq<FAST>(dv<FAST>(e, E)) * m;
@@ -208,26 +240,113 @@ __device__ __forceinline__ void load_nb3(int t, const int* __restrict__ nbr, int
     }
 }
 
-// Own cells and the 2-cell ring of a tile into LDS, STRIDE-wide rows (the reference's state layout, in LDS).
-template <class R>
-__device__ __forceinline__ void stage(R* __restrict__ lds, const R* __restrict__ g, const int* nb) {
-    for (int p = threadIdx.x; p < STRIDE2; p += NT) {
-        const int jj = p / STRIDE;
-        const int ii = p - jj * STRIDE;
-        const int j = jj - HALO, i = ii - HALO;
-        const int dj = j < 0 ? -1 : (j >= TILE ? 1 : 0);
-        const int di = i < 0 ? -1 : (i >= TILE ? 1 : 0);
-        const int tt = nb[(dj + 1) * 3 + di + 1];
-        lds[p] = g[(size_t)tt * TILE2 + (j - dj * TILE) * TILE + (i - di * TILE)];
+// LDS block of one array: 36 rows of LS entries; the tile's own cells at rows 2..33, columns 4..35 (16-byte
+// aligned rows), the 2-cell ring at rows 0,1 and 34,35 and columns 2,3 and 36,37.
+constexpr int LS = 40;
+constexpr int LROWS = TILE + 2 * HALO;
+constexpr int LSIZE = LROWS * LS;
+constexpr int NRING = STRIDE2 - TILE2;  // 272 ring entries
+
+__device__ __forceinline__ int lds_of(int j, int i) { return (j + HALO) * LS + i + 4; }
+
+// Ring entry q (0 .. 271) as (j, i) of the tile: north rows, south rows, west columns, east columns.
+__device__ __forceinline__ void ring_cell(int q, int& j, int& i) {
+    if (q < 72) {
+        j = q / 36 - 2;
+        i = q - (q / 36) * 36 - 2;
+    } else if (q < 144) {
+        q -= 72;
+        j = TILE + q / 36;
+        i = q - (q / 36) * 36 - 2;
+    } else if (q < 208) {
+        q -= 144;
+        j = q >> 1;
+        i = (q & 1) - 2;
+    } else {
+        q -= 208;
+        j = q >> 1;
+        i = TILE + (q & 1);
+    }
+}
+
+// What one thread loads into every staged array: 4 consecutive own cells and one or two ring entries; the same
+// offsets serve every array, so they are worked out once per workgroup.
+struct Stager {
+    int own_g, own_l;
+    int ring_g[2], ring_l[2];
+};
+
+__device__ __forceinline__ Stager make_stager(int t, const int* nb) {
+    Stager s;
+    const int q0 = threadIdx.x;
+    s.own_g = t * TILE2 + 4 * q0;
+    s.own_l = lds_of(q0 >> 3, (q0 & 7) * 4);
+#pragma unroll
+    for (int m = 0; m < 2; ++m) {
+        const int q = q0 + m * NT;
+        s.ring_l[m] = -1;
+        s.ring_g[m] = 0;
+        if (q < NRING) {
+            int j, i;
+            ring_cell(q, j, i);
+            const int dj = j < 0 ? -1 : (j >= TILE ? 1 : 0);
+            const int di = i < 0 ? -1 : (i >= TILE ? 1 : 0);
+            const int tt = nb[(dj + 1) * 3 + di + 1];
+            s.ring_g[m] = tt * TILE2 + (j - dj * TILE) * TILE + (i - di * TILE);
+            s.ring_l[m] = lds_of(j, i);
+        }
+    }
+    return s;
+}
+
+__device__ __forceinline__ void ld4(const float* __restrict__ g, float* o) {
+    const float4 v = *reinterpret_cast<const float4*>(g);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+    o[3] = v.w;
+}
+__device__ __forceinline__ void ld4(const double* __restrict__ g, double* o) {
+    const double2 a = *reinterpret_cast<const double2*>(g);
+    const double2 b = *reinterpret_cast<const double2*>(g + 2);
+    o[0] = a.x;
+    o[1] = a.y;
+    o[2] = b.x;
+    o[3] = b.y;
+}
+__device__ __forceinline__ void st4(float* l, const float* o) {
+    *reinterpret_cast<float4*>(l) = make_float4(o[0], o[1], o[2], o[3]);
+}
+__device__ __forceinline__ void st4(double* l, const double* o) {
+    *reinterpret_cast<double2*>(l) = make_double2(o[0], o[1]);
+    *reinterpret_cast<double2*>(l + 2) = make_double2(o[2], o[3]);
+}
+
+// N arrays into their LDS blocks: every load issued before the first store.
+template <class R, int N>
+__device__ __forceinline__ void stage(R* const (&lds)[N], const R* const (&g)[N], const Stager& s) {
+    R own[N][4], ring0[N], ring1[N];
+    const bool two = s.ring_l[1] >= 0;
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        ld4(g[a] + s.own_g, own[a]);
+        ring0[a] = g[a][s.ring_g[0]];
+        ring1[a] = two ? g[a][s.ring_g[1]] : R(0);
+    }
+#pragma unroll
+    for (int a = 0; a < N; ++a) {
+        st4(lds[a] + s.own_l, own[a]);
+        lds[a][s.ring_l[0]] = ring0[a];
+        if (two) lds[a][s.ring_l[1]] = ring1[a];
     }
 }
 
 // State index of the west / north neighbour of own cell (j, i) of tile t.
-__device__ __forceinline__ size_t west_of(int t, int j, int i, const int* nb) {
-    return i > 0 ? (size_t)t * TILE2 + j * TILE + i - 1 : (size_t)nb[3] * TILE2 + j * TILE + (TILE - 1);
+__device__ __forceinline__ unsigned west_of(int t, int j, int i, const int* nb) {
+    return i > 0 ? t * TILE2 + j * TILE + i - 1 : nb[3] * TILE2 + j * TILE + (TILE - 1);
 }
-__device__ __forceinline__ size_t north_of(int t, int j, int i, const int* nb) {
-    return j > 0 ? (size_t)t * TILE2 + (j - 1) * TILE + i : (size_t)nb[1] * TILE2 + (TILE - 1) * TILE + i;
+__device__ __forceinline__ unsigned north_of(int t, int j, int i, const int* nb) {
+    return j > 0 ? t * TILE2 + (j - 1) * TILE + i : nb[1] * TILE2 + (TILE - 1) * TILE + i;
 }
 
 template <class R>
@@ -239,14 +358,15 @@ struct FaceIn {
     const R* sh;
     const R* sd;
     int S, P;      // LDS steps along and across the normal
+    int ZS;        // step along the normal in the haloed centre-line bed arrays
 };
 
 // One face: the west (x) or north (y) face of own cell L of the LDS block; faces.cu:105-242 restated, with the
 // statement order of solver/cpu/kernels.py:86-219. Returns the wave speed.
 template <class R, bool FAST>
 __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st, R ef, R cw, const float* zc,
-                                  size_t zk, const float4* __restrict__ ftab, size_t plane, int last,
-                                  const R* __restrict__ om, size_t k_up, size_t k_own, R dt, bool want_wave,
+                                  unsigned zk, const float4* __restrict__ ftab, size_t plane, int last,
+                                  const R* __restrict__ om, unsigned k_up, unsigned k_own, R dt, bool want_wave,
                                   const Params& P, R& u_out, R& m_out, R& q_out) {
     const int S = in.S, Q = in.P;
     const R* u = in.su;
@@ -255,8 +375,8 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R eL = in.sd[L - S] - fl;
     const R eR = in.sd[L] - fr;
     const R uf = u[L];
-    const R eLp = pmax(eL, R(0));
-    const R eRp = pmax(eR, R(0));
+    const R eLp = fmx<FAST>(eL, R(0));
+    const R eRp = fmx<FAST>(eR, R(0));
     const bool from_left = uf > R(0) || (uf == R(0) && eL >= eR);
     const R eup = from_left ? eLp : eRp;
     if (eup <= R(P.e_wet)) {
@@ -267,7 +387,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     }
     R eface = eup;
     if (P.carry && cw == R(0)) {
-        const R dz = from_left ? R(zc[zk - S]) - fl : R(zc[zk]) - fr;
+        const R dz = from_left ? R(zc[zk - in.ZS]) - fl : R(zc[zk]) - fr;
         if (dz > R(1e-6)) {
             R phi = dv<FAST>(eup - R(2) * dz, R(2) * dz);
             phi = phi < R(0) ? R(0) : (phi > R(1) ? R(1) : phi);
@@ -287,7 +407,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
         const R Avol = hup * dx;
         const R edown = from_left ? eRp : eLp;
         if (Avol < A) {
-            const R s = dv<FAST>(pmin(eLp, eRp), pmax(eLp, eRp));
+            const R s = dv<FAST>(fmn<FAST>(eLp, eRp), fmx<FAST>(eLp, eRp));
             const R s2 = s * s;
             const R alpha = s2 * s2;
             rho = alpha + dv<FAST>((R(1) - alpha) * Avol, A);
@@ -297,9 +417,9 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
             const R gU = from_left ? gl : gr;
             const R gD = from_left ? gr : gl;
             const R hD = from_left ? hR : hL;
-            const R drop = (hup - gU) - pmax(hD - gD, R(0.5) * (gU - gD));
+            const R drop = (hup - gU) - fmx<FAST>(hD - gD, R(0.5) * (gU - gD));
             const R level_drop = fabs(eL - eR);
-            const R drive = pmax(eup, pmin(drop, level_drop));
+            const R drive = fmx<FAST>(eup, fmn<FAST>(drop, level_drop));
             dEta = from_left ? -drive : drive;
         }
     }
@@ -313,8 +433,10 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R* q = in.sq;
     const R QcL = R(0.5) * (q[L - S] + q[L]);
     const R QcR = R(0.5) * (q[L] + q[L + S]);
-    const R sL = QcL > R(0) ? ustar(um, umm, uf, second) : ustar(uf, up, um, second);
-    const R sR = QcR > R(0) ? ustar(uf, um, up, second) : ustar(up, upp, uf, second);
+    // each centre velocity: the upwind face's, corrected; the arguments are chosen first, one correction is made
+    const bool pL = QcL > R(0), pR = QcR > R(0);
+    const R sL = ustar(pL ? um : uf, pL ? umm : up, pL ? uf : um, second);
+    const R sR = ustar(pR ? uf : up, pR ? um : upp, pR ? up : uf, second);
     R flux = QcR * sR - QcL * sL;
     const R* qp = in.sqp;
     const R QyS = R(0.5) * (qp[L - S] + qp[L]);
@@ -323,8 +445,9 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R uN = gated(u[L + Q], uf, mirror);
     const R uSS = gated(u[L - 2 * Q], uS, mirror);
     const R uNN = gated(u[L + 2 * Q], uN, mirror);
-    const R usS = QyS > R(0) ? ustar(uS, uSS, uf, second) : ustar(uf, uN, uS, second);
-    const R usN = QyN > R(0) ? ustar(uf, uS, uN, second) : ustar(uN, uNN, uf, second);
+    const bool pS = QyS > R(0), pN = QyN > R(0);
+    const R usS = ustar(pS ? uS : uf, pS ? uSS : uN, pS ? uf : uS, second);
+    const R usN = ustar(pN ? uf : uN, pN ? uS : uNN, pN ? uN : uf, second);
     flux += QyN * usN - QyS * usS;
 
     // [4] momentum and pressure
@@ -340,8 +463,8 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R* vp = in.sv;
     const R vb = R(0.25) * (vp[L - S] + vp[L] + vp[L - S + Q] + vp[L + Q]);
     const R T3 = T * T * T * cb<FAST>(T);
-    R fric = dv<FAST>(A * A, pmax(T3, R(1e-30)));
-    if (rho < R(1)) fric = dv<FAST>(fric, pmax(rho * cb<FAST>(rho), R(1e-12)));
+    R fric = dv<FAST>(A * A, fmx<FAST>(T3, R(1e-30)));
+    if (rho < R(1)) fric = dv<FAST>(fric, fmx<FAST>(rho * cb<FAST>(rho), R(1e-12)));
     const R a = dt * g * fric;
     const R sp = sq<FAST>(us * us + vb * vb);
     R un = dv<FAST>(R(2) * us, R(1) + sq<FAST>(R(1) + R(4) * a * sp));
@@ -359,10 +482,10 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
             R Tc;
             face_lookup<R, FAST>(eup + dv<FAST>(R(0.5) * ua * ua, g), ef, ftab, plane, last, Ac, Tc, Wc);
         }
-        const R qc = c * Ac * sq<FAST>(dv<FAST>(Ac, pmax(Wc, R(1e-9))));
+        const R qc = c * Ac * sq<FAST>(dv<FAST>(Ac, fmx<FAST>(Wc, R(1e-9))));
         if (fabs(qn) > qc) {
             qn = qn > R(0) ? qc : -qc;
-            un = dv<FAST>(qn, pmax(A * rho, R(1e-12)));
+            un = dv<FAST>(qn, fmx<FAST>(A * rho, R(1e-12)));
         }
     }
     u_out = un;
@@ -378,9 +501,9 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     if (dup <= h_cfl) return R(0);
     R plan = from_left ? om[k_up] : om[k_own];
     const R dother = from_left ? dR : dL;
-    if (dother > h_cfl) plan = pmin(plan, from_left ? om[k_own] : om[k_up]);
-    const R wave = sq<FAST>(pmax(dv<FAST>(g * A * rho, pmax(plan, R(P.a_min)) * dx), R(0)));
-    return fabs(un) + pmin(R(P.lead) * fabs(un - uf), R(2) * wave) + wave;
+    if (dother > h_cfl) plan = fmn<FAST>(plan, from_left ? om[k_own] : om[k_up]);
+    const R wave = sq<FAST>(fmx<FAST>(dv<FAST>(g * A * rho, fmx<FAST>(plan, R(P.a_min)) * dx), R(0)));
+    return fabs(un) + fmn<FAST>(R(P.lead) * fabs(un - uf), R(2) * wave) + wave;
 }
 
 template <class R>
@@ -413,31 +536,36 @@ struct StepArgs {
 // Both own faces of every own cell of one active tile per workgroup (faces.cu:244-271).
 template <class R, bool FAST>
 __global__ __launch_bounds__(NT) void faces_kernel(StepArgs<R> A) {
-    __shared__ R s_u[STRIDE2], s_v[STRIDE2], s_h[STRIDE2], s_d[STRIDE2], s_qx[STRIDE2], s_qy[STRIDE2];
+    __shared__ __align__(16) R s_u[LSIZE];
+    __shared__ __align__(16) R s_v[LSIZE];
+    __shared__ __align__(16) R s_h[LSIZE];
+    __shared__ __align__(16) R s_d[LSIZE];
+    __shared__ __align__(16) R s_qx[LSIZE];
+    __shared__ __align__(16) R s_qy[LSIZE];
     __shared__ int nb[9];
     __shared__ R red[NT / 64];
     if ((int)blockIdx.x >= A.nact) return;
     const int t = A.active[blockIdx.x];
     load_nb3(t, A.st.nbr, nb);
     __syncthreads();
-    stage(s_u, A.s.u, nb);
-    stage(s_v, A.s.v, nb);
-    stage(s_h, A.s.h, nb);
-    stage(s_d, A.s.d, nb);
-    stage(s_qx, A.s.qx, nb);
-    stage(s_qy, A.s.qy, nb);
+    {
+        const Stager sg = make_stager(t, nb);
+        R* const dst[6] = {s_u, s_v, s_h, s_d, s_qx, s_qy};
+        const R* const src[6] = {A.s.u, A.s.v, A.s.h, A.s.d, A.s.qx, A.s.qy};
+        stage<R, 6>(dst, src, sg);
+    }
     __syncthreads();
-    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, STRIDE};
-    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, STRIDE, 1};
+    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, LS, 1};
+    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1, STRIDE};
     const int i = threadIdx.x & (TILE - 1);
     const bool want_wave = A.want_wave != 0;
     R w = R(0);
 #pragma unroll 1
     for (int r = 0; r < ROWS_PER_THREAD; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NT / TILE);
-        const int L = (j + HALO) * STRIDE + i + HALO;
-        const size_t c = (size_t)t * TILE2 + j * TILE + i;
-        const size_t zk = (size_t)t * STRIDE2 + L;
+        const int L = lds_of(j, i);
+        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+        const unsigned zk = (unsigned)(t * STRIDE2 + (j + HALO) * STRIDE + i + HALO);
         const float4 e = A.st.ec[c];
         R un, mn, qn;
         const R wx = face<R, FAST>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, zk, A.st.ftabx + c,
@@ -468,7 +596,10 @@ __device__ __forceinline__ R scaled(R q, R rl, R rr) { return q * (q > R(0) ? rl
 // residual, the level and wet share of every own cell (cells.cu:28-63).
 template <class R, bool FAST>
 __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
-    __shared__ R s_h[STRIDE2], s_qx[STRIDE2], s_qy[STRIDE2], s_r[STRIDE2];
+    __shared__ __align__(16) R s_h[LSIZE];
+    __shared__ __align__(16) R s_qx[LSIZE];
+    __shared__ __align__(16) R s_qy[LSIZE];
+    __shared__ __align__(16) R s_r[LSIZE];
     __shared__ int nb[9];
     __shared__ int any_wet;
     if ((int)blockIdx.x >= A.nact) return;
@@ -476,9 +607,12 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     load_nb3(t, A.st.nbr, nb);
     if (threadIdx.x == 0) any_wet = 0;
     __syncthreads();
-    stage(s_h, A.s.h, nb);
-    stage(s_qx, A.s.qx1, nb);
-    stage(s_qy, A.s.qy1, nb);
+    {
+        const Stager sg = make_stager(t, nb);
+        R* const dst[3] = {s_h, s_qx, s_qy};
+        const R* const src[3] = {A.s.h, A.s.qx1, A.s.qy1};
+        stage<R, 3>(dst, src, sg);
+    }
     __syncthreads();
     const R dx = R(A.P.dx);
     const R area = dx * dx;
@@ -486,8 +620,8 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32
     for (int p = threadIdx.x; p < RW * RW; p += NT) {
         const int jj = p / RW, ii = p - (p / RW) * RW;
-        const int L = (jj + 1) * STRIDE + ii + 1;
-        const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + STRIDE], R(0)) +
+        const int L = lds_of(jj - 1, ii - 1);
+        const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + LS], R(0)) +
                       pmax(-s_qy[L], R(0));
         const R have = pmax(s_h[L], R(0)) * area;
         s_r[L] = out * dt > have ? dv<FAST>(have, out * dt) : R(1);
@@ -499,13 +633,13 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
 #pragma unroll 1
     for (int r = 0; r < ROWS_PER_THREAD; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NT / TILE);
-        const int L = (j + HALO) * STRIDE + i + HALO;
-        const size_t c = (size_t)t * TILE2 + j * TILE + i;
+        const int L = lds_of(j, i);
+        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
         const R rc = s_r[L];
         const R qw = scaled(s_qx[L], s_r[L - 1], rc);
         const R qe = scaled(s_qx[L + 1], rc, s_r[L + 1]);
-        const R qn = scaled(s_qy[L], s_r[L - STRIDE], rc);
-        const R qs = scaled(s_qy[L + STRIDE], rc, s_r[L + STRIDE]);
+        const R qn = scaled(s_qy[L], s_r[L - LS], rc);
+        const R qs = scaled(s_qy[L + LS], rc, s_r[L + LS]);
         A.s.qx[c] = qw;
         A.s.qy[c] = qn;
         const R h0 = s_h[L];
@@ -679,6 +813,11 @@ template <class R>
 __global__ void scale_source_kernel(size_t n, const double* __restrict__ w, double s, R* __restrict__ src) {
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x)
         src[p] = (R)(w[p] * s);
+}
+
+// A plain copy, the yardstick for the memory counters: n float4 read once and written once.
+__global__ __launch_bounds__(256) void copy_probe_kernel(size_t n, const float4* __restrict__ a, float4* __restrict__ b) {
+    for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) b[p] = a[p];
 }
 
 template <class R>
@@ -944,6 +1083,10 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     H->ntiles = T->ntiles;
     H->rows = T->rows;
     H->ncell = (size_t)T->ntiles * TILE2;
+    if (H->ncell * 16 > (size_t)INT32_MAX) {
+        delete H;
+        return fail(1, "w2d_create: more cells than 32-bit cell offsets reach (at most 134,217,728)");
+    }
     H->P = Params{p->g, p->dx, p->e_wet, p->h_cfl, p->u_max, p->lead, p->a_min, T->rows, p->second, p->sheet,
                   p->mirror, p->carry};
     int s = 0;
@@ -1407,6 +1550,34 @@ int w2d_timing_read(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* 
     *faces_ms = f;
     *cells_ms = c;
     *launches = (int64_t)n;
+    return 0;
+}
+
+int w2d_copy_probe(int64_t bytes, int32_t repeats, double* ms) {
+    if (bytes < 16 || repeats < 1 || !ms) return fail(1, "copy_probe: bad arguments");
+    const size_t n = (size_t)bytes / 16;
+    float4 *a = nullptr, *b = nullptr;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipMalloc(&a, n * 16));
+    if (hipMalloc(&b, n * 16) != hipSuccess) {
+        (void)hipFree(a);
+        return fail(2, "copy_probe: out of memory");
+    }
+    HIP_TRY(hipMemset(a, 0, n * 16));
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    copy_probe_kernel<<<2048, 256>>>(n, a, b);
+    HIP_TRY(hipEventRecord(e0, nullptr));
+    for (int r = 0; r < repeats; ++r) copy_probe_kernel<<<2048, 256>>>(n, a, b);
+    HIP_TRY(hipEventRecord(e1, nullptr));
+    HIP_TRY(hipEventSynchronize(e1));
+    float t = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+    *ms = t / repeats;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
     return 0;
 }
 

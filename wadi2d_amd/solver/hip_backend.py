@@ -66,6 +66,7 @@ _SIGS = {
     "w2d_source_sum": [VP, ctypes.POINTER(D)],
     "w2d_device_count": [ctypes.POINTER(I32)],
     "w2d_timing": [VP, I32],
+    "w2d_copy_probe": [I64, I32, ctypes.POINTER(D)],
     "w2d_timing_read": [VP, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(I64)],
 }
 
@@ -98,6 +99,13 @@ def device_count() -> int:
     n = I32(0)
     check(api().w2d_device_count(ctypes.byref(n)))
     return int(n.value)
+
+
+def copy_probe(nbytes: int, repeats: int = 10) -> float:
+    """Milliseconds of one plain device copy of ``nbytes`` (mean of ``repeats``)."""
+    ms = D(0.0)
+    check(api().w2d_copy_probe(int(nbytes), int(repeats), ctypes.byref(ms)))
+    return float(ms.value)
 
 
 def _c(a: np.ndarray, dtype) -> np.ndarray:
