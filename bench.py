@@ -5,7 +5,9 @@ Workload: the full-extent plain FP32 step, 4096 x 4096 cells at 30 m (16,777,216
 (``wadi2d_amd.synth.full_extent_case``). The run is taken hot to t = 1.5 h (the storm's peak) through the same
 window loop a run uses, then W steps of warm-up and exactly K timed steps. A step is what the stepper does per
 step: both kernels, the boundary cells, the FP64 boundary tally, a wave-speed reduction every 8 steps, the
-active-tile check every 16 steps and a source refresh every 30 s of simulated time.
+active-tile check every 16 steps and a source refresh every 30 s of simulated time. By default the loop runs on
+the device (engine/device_run.py: captured graphs of 64 steps, no host in the loop); ``--loop host`` drives the same
+steps from Python through the backend protocol.
 
 Printed: one JSON line. ``value`` counts domain cells (padding excluded) times steps over the wall time of the
 timed region, with the inputs resident in HBM. ``roofline`` is the face kernel's (the dominant one): its
@@ -49,6 +51,10 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample length; 0 skips it")
     ap.add_argument("--events", choices=("on", "off"), default="on", help="HIP events around the kernels")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="MiB of a plain device copy to time first")
+    ap.add_argument("--loop", choices=("device", "host"), default="device")
+    ap.add_argument("--graph", type=int, default=64, help="steps per captured graph (device loop)")
+    ap.add_argument("--face-cache", choices=("on", "off"), default="on", help="per-face row cache of the face tables")
+    ap.add_argument("--faces-threads", type=int, choices=(256, 512), default=512, help="face kernel workgroup size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -61,7 +67,7 @@ def main() -> int:
     if args.copy_probe > 0:
         nbytes = int(args.copy_probe * 2**20)
         ms = copy_probe(nbytes, 10)
-        log(f"[bench] copy probe: {nbytes} B read + {nbytes} B written in {ms:.4f} ms = {2 * nbytes / ms / 1e9:.0f} GB/s")
+        log(f"[bench] copy probe: {nbytes} B read + {nbytes} B written in {ms:.4f} ms = {2 * nbytes / ms / 1e6:.0f} GB/s")
     from wadi2d_amd.engine.forcing import Rain, boundary_set
     from wadi2d_amd.solver.model import Model
 
@@ -76,30 +82,57 @@ def main() -> int:
     del scen.bed
     backend = "hip" if args.variant == "fast" else "hip-exact"
     t0 = time.time()
-    model = Model(tables, scen.numerics, backend)
+    model = Model(tables, scen.numerics, backend, face_cache=args.face_cache == "on")
     model.set_depth(np.zeros(tables.grid.shape))
+    model.backend.set_launch(args.faces_threads)
     drv = Driver(model, bset, rain)
     log(f"[bench] device upload: {time.time() - t0:.1f} s")
     b = model.backend
     st = model.stepper
     t0 = time.time()
     hot = args.hot
-    while st.t < hot - 1e-9:
-        drv.run(min(st.t + 600.0, hot))
-        log(f"[bench] hot start t={st.t:.0f} s steps={st.nsteps} dt={st.dt:.3f} active={b.nact}")
-    log(f"[bench] hot start: {time.time() - t0:.1f} s")
-    drv.run(1e12, max_steps=st.nsteps + args.warmup)
-    b.synchronize()
-    b.timing_read()
-    b.timing(args.events == "on")
-    n0, tsim0 = st.nsteps, st.t
-    t0 = time.perf_counter()
-    drv.run(1e12, max_steps=st.nsteps + args.steps)
-    b.synchronize()
-    wall = time.perf_counter() - t0
-    b.timing(False)
-    faces_ms, cells_ms, launches = b.timing_read()
-    steps = st.nsteps - n0
+    if args.loop == "host":
+        while st.t < hot - 1e-9:
+            drv.run(min(st.t + 600.0, hot))
+            log(f"[bench] hot start t={st.t:.0f} s steps={st.nsteps} dt={st.dt:.3f} active={b.nact}")
+        log(f"[bench] hot start: {time.time() - t0:.1f} s")
+        drv.run(1e12, max_steps=st.nsteps + args.warmup)
+        b.synchronize()
+        b.timing_read()
+        b.timing(args.events == "on")
+        n0, tsim0 = st.nsteps, st.t
+        t0 = time.perf_counter()
+        drv.run(1e12, max_steps=st.nsteps + args.steps)
+        b.synchronize()
+        wall = time.perf_counter() - t0
+        b.timing(False)
+        faces_ms, cells_ms, launches = b.timing_read()
+        steps, tsim = st.nsteps - n0, st.t - tsim0
+        bal = drv.accounts()
+    else:
+        from wadi2d_amd.engine.device_run import DeviceRun, MemberPlan
+
+        plan = MemberPlan(1, tables.grid.ntiles, rain, bset, st.t, scen.duration_s, st.dt, st.nsteps,
+                          st.refresh_due, rain.state)  # fmt: skip
+        run = DeviceRun(b, [plan], scen.numerics, idle=st.idle)
+        state = run.state[0]
+        while state.t < hot - 1e-9:
+            state = run.steps(256, args.graph)[0]
+            if state.nsteps % 1024 == 0:
+                log(f"[bench] hot start t={state.t:.0f} s steps={state.nsteps} dt={state.dt:.3f} active={b.nact}")
+        log(f"[bench] hot start: {time.time() - t0:.1f} s, t={state.t:.0f} s, {state.nsteps} steps")
+        run.steps(args.warmup, args.graph)
+        b.timing_read()
+        b.timing(args.events == "on")
+        n0, tsim0 = run.state[0].nsteps, run.state[0].t
+        t0 = time.perf_counter()
+        state = run.steps(args.steps, args.graph)[0]  # waits for the last step
+        wall = time.perf_counter() - t0
+        b.timing(False)
+        faces_ms, cells_ms, launches = b.timing_read()
+        steps, tsim = state.nsteps - n0, state.t - tsim0
+        assert not state.done, "the run ended inside the timed steps"
+        bal = run.balance(0, b.tally_volume())
     assert steps == args.steps and launches in (0, steps), (steps, launches)
     launches = max(launches, 1)
     domain_cells = int(tables.grid.mask.sum())
@@ -108,14 +141,13 @@ def main() -> int:
     face_s = faces_ms / 1e3 / launches
     cell_s = cells_ms / 1e3 / launches
     face_achieved = FACE_BYTES * tiles * 1024 / face_s if face_s > 0 else 0.0
-    bal = drv.accounts()
     log(f"[bench] timed {steps} steps in {wall:.3f} s ({wall / steps * 1e3:.3f} ms/step), simulated "
-        f"{st.t - tsim0:.0f} s; faces {face_s * 1e3:.3f} ms, cells {cell_s * 1e3:.3f} ms per launch; "
+        f"{tsim:.0f} s; faces {face_s * 1e3:.3f} ms, cells {cell_s * 1e3:.3f} ms per launch; "
         f"active tiles {tiles}; mass error {bal.relative:.2e}")
 
     cpu = None
     if args.cpu_seconds > 0:
-        cpu = cpu_baseline(model, tables, scen.numerics, args.cpu_seconds)
+        cpu = cpu_baseline(model, tables, scen.numerics, args.cpu_seconds, st.dt if args.loop == "host" else state.dt)
 
     out = {
         "metric": "cell-updates/s",
@@ -136,6 +168,7 @@ def main() -> int:
             "cells": domain_cells,
             "active_tiles": tiles,
             "variant": args.variant,
+            "loop": args.loop,
             "hbm_frac_step": STEP_BYTES * value / HBM_PEAK,
             "faces_ms": face_s * 1e3,
             "cells_ms": cell_s * 1e3,
@@ -156,7 +189,7 @@ def main() -> int:
     return 0
 
 
-def cpu_baseline(model, tables, num, seconds: float):
+def cpu_baseline(model, tables, num, seconds: float, dt: float):
     """The oracle's float32 step on the host's cores, from the GPU's hot state, for about ``seconds``."""
     from oracle.oracle import Numerics as ONumerics
     from oracle.oracle import OracleBackend
@@ -168,7 +201,6 @@ def cpu_baseline(model, tables, num, seconds: float):
         ob.set_field(f, model.backend.get_field(f))
     ob.set_boundary(model.backend.bc)
     log(f"[bench] cpu baseline set up in {time.time() - t0:.1f} s on {cores} threads")
-    dt = float(model.stepper.dt)
     cells = int(tables.grid.mask.sum())
     done, spent = 0, 0.0
     while spent < seconds:

@@ -43,6 +43,9 @@ enum { W2D_F32 = 4, W2D_F64 = 8 };
 /* W2D_EXACT: IEEE division, square root and cube root, no contraction: the float32 statement order of
  * solver/cpu/kernels.py. W2D_FAST: hardware reciprocal, square root and log/exp cube root in float32. */
 enum { W2D_EXACT = 0, W2D_FAST = 1 };
+/* Or-ed into the variant: face-table lookups gather from the table every time instead of through the per-face
+ * row cache (the results are the same; only the memory traffic differs). */
+enum { W2D_NO_FACE_CACHE = 4 };
 
 /* Field ids: layout.STATE_ARRAYS then layout.WORK_ARRAYS (layout.py:25-29). */
 enum {
@@ -85,6 +88,8 @@ int w2d_refresh_halos(w2d_handle* h);
 int w2d_tally_set(w2d_handle* h, const int32_t* face, const int32_t* axis, const float* sign, int32_t n);
 int w2d_tally_add(w2d_handle* h, double dt);
 int w2d_tally_volume(w2d_handle* h, double* out);
+/* Every face's own running total (n as set), for books per ensemble member. */
+int w2d_tally_totals(w2d_handle* h, double* out);
 /* The instantaneous flow through the tally's faces, sum of sign * q, m3/s (no counterpart: the outfall discharge). */
 int w2d_tally_flow(w2d_handle* h, double* out);
 /* Maxima (engine/maxima.py:28-51,103-146): depth, mean depth, speed, hazard, arrival; which = 0..4. */
@@ -104,9 +109,47 @@ int w2d_source_sum(w2d_handle* h, double* out);
  * face and cell kernels; timing_read waits, and returns and clears the summed milliseconds and the step count. */
 int w2d_timing(w2d_handle* h, int32_t on);
 int w2d_timing_read(w2d_handle* h, double* faces_ms, double* cells_ms, int64_t* launches);
+int w2d_timing_read_marks(w2d_handle* h, double* faces_ms, double* cells_ms, int64_t* launches);
 /* A plain device copy of `bytes` (no counterpart): the yardstick that scales the memory counters; returns the mean
  * milliseconds per copy over `repeats` copies. */
 int w2d_copy_probe(int64_t bytes, int32_t repeats, double* ms);
+/* Device-resident run (no counterpart: it is the loop of Stepper.advance, solver/stepper.py:101-129, and of the
+ * run's source windows, run.py:1712-1739, moved onto the device, for one model or an ensemble). Members own
+ * contiguous tile ranges of the handle; each keeps its own clock, step length, wave refresh, source windows, rain
+ * and active tiles. Window k of a member spans [t0 + k window, min(t0 + (k+1) window, t_end)]; scales[offset + k] is
+ * its source rate per unit weight (see w2d_source_weights), m/s. */
+typedef struct w2d_member {
+    int32_t tile0, tile1;
+    double t0, t_end, dt0, window;
+    int32_t nwindows;
+    int64_t scale_offset;
+    int64_t nsteps0;
+    int32_t refresh_due;
+} w2d_member;
+typedef struct w2d_run_numerics {
+    double cfl, dt_max, dt_min, wave_floor;
+    int32_t dt_every, active_every, maxima;
+} w2d_run_numerics;
+typedef struct w2d_member_state {
+    double t, dt, rain;
+    int64_t nsteps;
+    int32_t done, failed, refresh_due;
+} w2d_member_state;
+/* Set up a run from the handle's current state, active list and the stepper's idle counts (ntiles or null). */
+int w2d_run_setup(w2d_handle* h, const w2d_member* members, int32_t M, const double* scales, int64_t nscales,
+                  const w2d_run_numerics* num, const int32_t* idle);
+/* Enqueue nsteps steps of every live member; with graph_steps > 0 (even) they are replayed from one captured graph
+ * of that many steps (not while kernel timing is on: HIP does not time events inside graphs). Nothing waits. */
+int w2d_run_steps(w2d_handle* h, int32_t nsteps, int32_t graph_steps);
+/* Wait, and read every member's clock, step, count, rain volume and flags (failed: 1 non-finite wave, 2 step below
+ * the minimum). */
+int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
+int w2d_run_idle(w2d_handle* h, int32_t* out);
+/* The active list as the device last left it (out may be null to ask for the count). */
+int w2d_active_tiles(w2d_handle* h, int32_t* out, int32_t* n);
+/* Workgroup size of the face kernel, 256 or 512 threads (default 512); the results do not depend on it. No
+ * counterpart in the reference, whose launch shape is fixed (gpu/backend.py:151-188): a tuning knob. */
+int w2d_set_launch(w2d_handle* h, int32_t faces_threads);
 /* synchronize (gpu/backend.py:202-203). */
 int w2d_synchronize(w2d_handle* h);
 const char* w2d_last_error(void);

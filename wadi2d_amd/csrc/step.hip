@@ -44,7 +44,7 @@ constexpr int HALO = 2;
 constexpr int STRIDE = TILE + 2 * HALO;  // 36: the reference's state row, and the LDS row here
 constexpr int STRIDE2 = STRIDE * STRIDE;
 constexpr int NT = 256;                   // threads per workgroup: one tile
-constexpr int ROWS_PER_THREAD = TILE2 / NT;
+constexpr int ROWS_PER_THREAD = TILE2 / NT;  // the cell kernel's
 
 thread_local std::string g_error;
 
@@ -139,15 +139,32 @@ struct Params {
     int rows, second, sheet, mirror, carry;
 };
 
+// The row pair `r` of a table record, through the face's (or cell's) row cache when it has one: the cache holds
+// the pair of the row last read and its row in `tag`, so a lookup whose row has not changed since the last step
+// reads 17 coalesced bytes instead of gathering a 128-byte line; a row change reads the table and refills it.
+template <class V>
+__device__ __forceinline__ V fetch_row(const V* __restrict__ tab, size_t plane, int r, V* cache, unsigned char* tag,
+                                       int have) {
+    if (cache) {
+        if (have == r) return *cache;
+        const V q = tab[(size_t)r * plane];
+        *cache = q;
+        *tag = (unsigned char)r;
+        return q;
+    }
+    return tab[(size_t)r * plane];
+}
+
 // Area, conveyance^(3/5), wet width at depth e over the invert (common.cuh:64-90): rows quadratic in e, read
 // linearly between them, prismatic above the table top E. `tab` points at this face in plane 0; planes are
-// `plane` records apart.
+// `plane` records apart. `cache`/`tag`/`have`: the face's row cache (null for none), `have` its tag as read.
 template <class R, bool FAST>
 __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__ tab, size_t plane, int last, R& A,
-                                            R& T, R& W) {
+                                            R& T, R& W, float4* cache = nullptr, unsigned char* tag = nullptr,
+                                            int have = -1) {
     const R m = R(last);
     if (e >= E) {
-        const float4 q = tab[(size_t)(last - 1) * plane];
+        const float4 q = fetch_row(tab, plane, last - 1, cache, tag, have);
         const R a1 = R(q.z), a0 = R(q.x);
         const R c = dv<FAST>(m - R(1), m);
         W = dv<FAST>(a1 - a0, E * (R(1) - c * c));
@@ -160,7 +177,7 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
         const R x = e * __builtin_amdgcn_rcpf(E);
         const R s = sq<FAST>(x) * m;
         const int r = row_of(s, last - 1);
-        const float4 q = tab[(size_t)r * plane];
+        const float4 q = fetch_row(tab, plane, r, cache, tag, have);
         const R im = __builtin_amdgcn_rcpf(m);
         const R c0 = R(r) * im;
         const R c1 = R(r + 1) * im;
@@ -176,7 +193,7 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
     }
     const R s = sq<FAST>(dv<FAST>(e, E)) * m;
     const int r = row_of(s, last - 1);
-    const float4 q = tab[(size_t)r * plane];
+    const float4 q = fetch_row(tab, plane, r, cache, tag, have);
     const R c0 = dv<FAST>(R(r), m);
     const R c1 = dv<FAST>(R(r + 1), m);
     const R e0 = c0 * c0 * E;
@@ -186,6 +203,28 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
     A = a0 + f * (a1 - a0);
     T = t0 + f * (R(q.w) - t0);
     W = dv<FAST>(a1 - a0, de);
+}
+
+// Level and, when asked, wet share of a cell from one row pair read through the cell's row cache; the same
+// statements as cell_level and cell_plan below.
+template <class R, bool FAST>
+__device__ __forceinline__ R cell_eval(R hb, R Hc, const float2* __restrict__ tab, size_t plane, int last,
+                                       float2* cache, unsigned char* tag, bool want_plan, R& plan) {
+    plan = R(1);
+    if (Hc <= R(0)) return hb;
+    if (hb >= Hc) {
+        const int have = *tag;
+        return R(fetch_row(tab, plane, last - 1, cache, tag, have).y) + (hb - Hc);
+    }
+    const int have = *tag;
+    const R s = sq<FAST>(dv<FAST>(hb, Hc)) * R(last);
+    const int r = row_of(s, last - 1);
+    const float2 q = fetch_row(tab, plane, r, cache, tag, have);
+    if (want_plan) {
+        const R m = R(last);
+        plan = dv<FAST>(dv<FAST>(Hc * R(2 * r + 1), m * m), pmax(R(q.y) - R(q.x), R(1e-12)));
+    }
+    return R(q.x) + (s - R(r)) * (R(q.y) - R(q.x));
 }
 
 // Level above zmin from the mean depth, and the wet share of the plan (common.cuh:39-56).
@@ -225,12 +264,28 @@ struct Statics {
     const float4* sx;         // (fl, fr, gl, gr) of x faces
     const float4* sy;
     const float4* ec;         // (efx, cwx, efy, cwy)
-    const float* zcx;         // state layout with halos (ntiles * 1296), filled once
+    const float* zcx;         // ncell, own layout; a neighbour tile's value is read from that tile
     const float* zcy;
+    float4* fcx;              // row caches of the face tables and their row tags (255: empty)
+    float4* fcy;
+    unsigned char* ftagx;
+    unsigned char* ftagy;
+    float2* cc;               // row cache of the cell tables
+    unsigned char* ctag;
     const int* nbr;           // (ntiles, 8)
     size_t ncell;
     int last;                 // rows - 1
 };
+
+// Workgroups go round-robin over the 8 XCDs, each with its own L2. Block b takes slot k = b / 8 of its XCD's
+// contiguous share of the n listed tiles, so a tile's east and west neighbours (the next and previous entries of the
+// list) are read by the same L2 at about the same time; -1 for a block with no slot.
+constexpr int XCDS = 8;
+__device__ __forceinline__ int xcd_slot(int b, int n) {
+    const int x = b % XCDS, k = b / XCDS, q = n / XCDS, r = n % XCDS;
+    if (k >= q + (x < r ? 1 : 0)) return -1;
+    return x * q + (x < r ? x : r) + k;
+}
 
 // The 3 x 3 block of tiles around t, (dj + 1) * 3 + (di + 1); NEIGHBOURS order of grid/tiles.py:12.
 __device__ __forceinline__ void load_nb3(int t, const int* __restrict__ nbr, int* nb) {
@@ -269,21 +324,26 @@ __device__ __forceinline__ void ring_cell(int q, int& j, int& i) {
     }
 }
 
-// What one thread loads into every staged array: 4 consecutive own cells and one or two ring entries; the same
-// offsets serve every array, so they are worked out once per workgroup.
+// What one thread loads into every staged array: OWN = 1024 / NTH consecutive own cells and up to RING ring
+// entries; the same offsets serve every array, so they are worked out once per workgroup.
+template <int NTH>
 struct Stager {
+    static constexpr int OWN = TILE2 / NTH;
+    static constexpr int RING = (NRING + NTH - 1) / NTH;
     int own_g, own_l;
-    int ring_g[2], ring_l[2];
+    int ring_g[RING], ring_l[RING];
 };
 
-__device__ __forceinline__ Stager make_stager(int t, const int* nb) {
-    Stager s;
+template <int NTH>
+__device__ __forceinline__ Stager<NTH> make_stager(int t, const int* nb) {
+    Stager<NTH> s;
     const int q0 = threadIdx.x;
-    s.own_g = t * TILE2 + 4 * q0;
-    s.own_l = lds_of(q0 >> 3, (q0 & 7) * 4);
+    const int p = Stager<NTH>::OWN * q0;
+    s.own_g = t * TILE2 + p;
+    s.own_l = lds_of(p >> 5, p & (TILE - 1));
 #pragma unroll
-    for (int m = 0; m < 2; ++m) {
-        const int q = q0 + m * NT;
+    for (int m = 0; m < Stager<NTH>::RING; ++m) {
+        const int q = q0 + m * NTH;
         s.ring_l[m] = -1;
         s.ring_g[m] = 0;
         if (q < NRING) {
@@ -299,45 +359,56 @@ __device__ __forceinline__ Stager make_stager(int t, const int* nb) {
     return s;
 }
 
-__device__ __forceinline__ void ld4(const float* __restrict__ g, float* o) {
-    const float4 v = *reinterpret_cast<const float4*>(g);
-    o[0] = v.x;
-    o[1] = v.y;
-    o[2] = v.z;
-    o[3] = v.w;
+// W consecutive values, global -> registers -> LDS, as 16-byte (or 8-byte for two floats) accesses.
+template <int W>
+__device__ __forceinline__ void ldw(const float* __restrict__ g, float* o) {
+    if constexpr (W == 4) {
+        const float4 v = *reinterpret_cast<const float4*>(g);
+        o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
+    } else {
+        static_assert(W == 2);
+        const float2 v = *reinterpret_cast<const float2*>(g);
+        o[0] = v.x, o[1] = v.y;
+    }
 }
-__device__ __forceinline__ void ld4(const double* __restrict__ g, double* o) {
-    const double2 a = *reinterpret_cast<const double2*>(g);
-    const double2 b = *reinterpret_cast<const double2*>(g + 2);
-    o[0] = a.x;
-    o[1] = a.y;
-    o[2] = b.x;
-    o[3] = b.y;
+template <int W>
+__device__ __forceinline__ void ldw(const double* __restrict__ g, double* o) {
+#pragma unroll
+    for (int k = 0; k < W; k += 2) {
+        const double2 a = *reinterpret_cast<const double2*>(g + k);
+        o[k] = a.x, o[k + 1] = a.y;
+    }
 }
-__device__ __forceinline__ void st4(float* l, const float* o) {
-    *reinterpret_cast<float4*>(l) = make_float4(o[0], o[1], o[2], o[3]);
+template <int W>
+__device__ __forceinline__ void stw(float* l, const float* o) {
+    if constexpr (W == 4)
+        *reinterpret_cast<float4*>(l) = make_float4(o[0], o[1], o[2], o[3]);
+    else
+        *reinterpret_cast<float2*>(l) = make_float2(o[0], o[1]);
 }
-__device__ __forceinline__ void st4(double* l, const double* o) {
-    *reinterpret_cast<double2*>(l) = make_double2(o[0], o[1]);
-    *reinterpret_cast<double2*>(l + 2) = make_double2(o[2], o[3]);
+template <int W>
+__device__ __forceinline__ void stw(double* l, const double* o) {
+#pragma unroll
+    for (int k = 0; k < W; k += 2) *reinterpret_cast<double2*>(l + k) = make_double2(o[k], o[k + 1]);
 }
 
 // N arrays into their LDS blocks: every load issued before the first store.
-template <class R, int N>
-__device__ __forceinline__ void stage(R* const (&lds)[N], const R* const (&g)[N], const Stager& s) {
-    R own[N][4], ring0[N], ring1[N];
-    const bool two = s.ring_l[1] >= 0;
+template <class R, int N, int NTH>
+__device__ __forceinline__ void stage(R* const (&lds)[N], const R* const (&g)[N], const Stager<NTH>& s) {
+    constexpr int W = Stager<NTH>::OWN, RN = Stager<NTH>::RING;
+    R own[N][W], ring[N][RN];
 #pragma unroll
     for (int a = 0; a < N; ++a) {
-        ld4(g[a] + s.own_g, own[a]);
-        ring0[a] = g[a][s.ring_g[0]];
-        ring1[a] = two ? g[a][s.ring_g[1]] : R(0);
+        ldw<W>(g[a] + s.own_g, own[a]);
+#pragma unroll
+        for (int m = 0; m < RN; ++m) ring[a][m] = s.ring_l[m] >= 0 ? g[a][s.ring_g[m]] : R(0);
     }
 #pragma unroll
     for (int a = 0; a < N; ++a) {
-        st4(lds[a] + s.own_l, own[a]);
-        lds[a][s.ring_l[0]] = ring0[a];
-        if (two) lds[a][s.ring_l[1]] = ring1[a];
+        stw<W>(lds[a] + s.own_l, own[a]);
+#pragma unroll
+        for (int m = 0; m < RN; ++m)
+            if (s.ring_l[m] >= 0) lds[a][s.ring_l[m]] = ring[a][m];
     }
 }
 
@@ -358,16 +429,15 @@ struct FaceIn {
     const R* sh;
     const R* sd;
     int S, P;      // LDS steps along and across the normal
-    int ZS;        // step along the normal in the haloed centre-line bed arrays
 };
 
 // One face: the west (x) or north (y) face of own cell L of the LDS block; faces.cu:105-242 restated, with the
 // statement order of solver/cpu/kernels.py:86-219. Returns the wave speed.
 template <class R, bool FAST>
 __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st, R ef, R cw, const float* zc,
-                                  unsigned zk, const float4* __restrict__ ftab, size_t plane, int last,
-                                  const R* __restrict__ om, unsigned k_up, unsigned k_own, R dt, bool want_wave,
-                                  const Params& P, R& u_out, R& m_out, R& q_out) {
+                                  const float4* __restrict__ ftab, size_t plane, int last, float4* cache,
+                                  unsigned char* tag, const R* __restrict__ om, unsigned k_up, unsigned k_own, R dt,
+                                  bool want_wave, const Params& P, R& u_out, R& m_out, R& q_out) {
     const int S = in.S, Q = in.P;
     const R* u = in.su;
     const R g = R(P.g), dx = R(P.dx);
@@ -385,9 +455,10 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
         q_out = R(0);
         return R(0);
     }
+    const int have = cache ? (int)*tag : -1;  // issued early: the row is known only after the carry
     R eface = eup;
     if (P.carry && cw == R(0)) {
-        const R dz = from_left ? R(zc[zk - in.ZS]) - fl : R(zc[zk]) - fr;
+        const R dz = from_left ? R(zc[k_up]) - fl : R(zc[k_own]) - fr;
         if (dz > R(1e-6)) {
             R phi = dv<FAST>(eup - R(2) * dz, R(2) * dz);
             phi = phi < R(0) ? R(0) : (phi > R(1) ? R(1) : phi);
@@ -395,7 +466,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
         }
     }
     R A, T, W;
-    face_lookup<R, FAST>(eface, ef, ftab, plane, last, A, T, W);
+    face_lookup<R, FAST>(eface, ef, ftab, plane, last, A, T, W, cache, tag, have);
     const R hL = in.sh[L - S];
     const R hR = in.sh[L];
     const R hup = from_left ? hL : hR;
@@ -506,7 +577,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     return fabs(un) + fmn<FAST>(R(P.lead) * fabs(un - uf), R(2) * wave) + wave;
 }
 
-template <class R>
+template <int NTH, class R>
 __device__ __forceinline__ R block_max(R v, R* red) {
     for (int off = 32; off > 0; off >>= 1) {
         const R o = __shfl_xor(v, off);
@@ -516,9 +587,34 @@ __device__ __forceinline__ R block_max(R v, R* red) {
     if ((threadIdx.x & 63) == 0) red[wid] = v;
     __syncthreads();
     R m = red[0];
-    for (int w = 1; w < NT / 64; ++w) m = red[w] > m ? red[w] : m;
+    for (int w = 1; w < NTH / 64; ++w) m = red[w] > m ? red[w] : m;
     return m;
 }
+
+// One member of a device-resident run: a model with its own clock, step, source windows and books. A plain run
+// is one member; an ensemble is many, their tiles side by side in one set of arrays (tile0 .. tile1).
+struct Member {
+    double t, t_end, dt, dt_step;
+    double window_start, window_end, window_len, scale;
+    double rain_done, rain_window;  // FP64 rain volume: whole past windows, and the current window
+    long long nsteps;
+    long long scale_off;
+    int nwin, win;
+    int tile0, tile1;
+    int b0, b1, f0, f1;  // the member's boundary cells and tally faces in the handle's lists
+    int stepping, refresh, refresh_due, new_window, dry, rebuild, check, done, failed;
+};
+
+struct RunGlobals {
+    int nact;
+    int any_rebuild;
+    int n_woken, n_leaving;
+};
+
+struct RunNum {
+    double cfl, dt_max, dt_min, wave_floor, dx, area;
+    int dt_every, active_every;
+};
 
 template <class R>
 struct StepArgs {
@@ -531,11 +627,17 @@ struct StepArgs {
     int want_wave;
     R* wave;     // per tile
     int* wet;    // per tile
+    // device-resident run (null in a protocol step): each tile's member says its step, flags and source
+    const Member* members;
+    const int* tile_member;
+    const RunGlobals* glob;
+    const double* src_weight;
+    double* src_partial;
 };
 
-// Both own faces of every own cell of one active tile per workgroup (faces.cu:244-271).
-template <class R, bool FAST>
-__global__ __launch_bounds__(NT) void faces_kernel(StepArgs<R> A) {
+// Both own faces of every own cell of one active tile per workgroup of NTH threads (faces.cu:244-271).
+template <class R, bool FAST, int NTH>
+__global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     __shared__ __align__(16) R s_u[LSIZE];
     __shared__ __align__(16) R s_v[LSIZE];
     __shared__ __align__(16) R s_h[LSIZE];
@@ -543,40 +645,49 @@ __global__ __launch_bounds__(NT) void faces_kernel(StepArgs<R> A) {
     __shared__ __align__(16) R s_qx[LSIZE];
     __shared__ __align__(16) R s_qy[LSIZE];
     __shared__ int nb[9];
-    __shared__ R red[NT / 64];
-    if ((int)blockIdx.x >= A.nact) return;
-    const int t = A.active[blockIdx.x];
+    __shared__ R red[NTH / 64];
+    const int slot = xcd_slot(blockIdx.x, A.members ? A.glob->nact : A.nact);
+    if (slot < 0) return;
+    const int t = A.active[slot];
+    R dt_run = A.dt;
+    int want_run = A.want_wave;
+    if (A.members) {
+        const Member& mb = A.members[A.tile_member[t]];
+        if (!mb.stepping) return;
+        dt_run = (R)mb.dt_step;
+        want_run = mb.refresh;
+    }
     load_nb3(t, A.st.nbr, nb);
     __syncthreads();
     {
-        const Stager sg = make_stager(t, nb);
+        const Stager<NTH> sg = make_stager<NTH>(t, nb);
         R* const dst[6] = {s_u, s_v, s_h, s_d, s_qx, s_qy};
         const R* const src[6] = {A.s.u, A.s.v, A.s.h, A.s.d, A.s.qx, A.s.qy};
-        stage<R, 6>(dst, src, sg);
+        stage<R, 6, NTH>(dst, src, sg);
     }
     __syncthreads();
-    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, LS, 1};
-    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1, STRIDE};
+    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, LS};
+    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1};
     const int i = threadIdx.x & (TILE - 1);
-    const bool want_wave = A.want_wave != 0;
+    const bool want_wave = want_run != 0;
+    const R dt = dt_run;
     R w = R(0);
 #pragma unroll 1
-    for (int r = 0; r < ROWS_PER_THREAD; ++r) {
-        const int j = (threadIdx.x >> 5) + r * (NT / TILE);
+    for (int r = 0; r < TILE2 / NTH; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
-        const unsigned zk = (unsigned)(t * STRIDE2 + (j + HALO) * STRIDE + i + HALO);
         const float4 e = A.st.ec[c];
         R un, mn, qn;
-        const R wx = face<R, FAST>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, zk, A.st.ftabx + c,
-                                   A.st.ncell, A.st.last, A.s.om, want_wave ? west_of(t, j, i, nb) : c, c, A.dt,
-                                   want_wave, A.P, un, mn, qn);
+        const R wx = face<R, FAST>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
+                                   A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c, A.s.om,
+                                   west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn);
         A.s.u1[c] = un;
         A.s.mx1[c] = mn;
         A.s.qx1[c] = qn;
-        const R wy = face<R, FAST>(fy, L, A.s.my[c], A.st.sy[c], R(e.z), R(e.w), A.st.zcy, zk, A.st.ftaby + c,
-                                   A.st.ncell, A.st.last, A.s.om, want_wave ? north_of(t, j, i, nb) : c, c, A.dt,
-                                   want_wave, A.P, un, mn, qn);
+        const R wy = face<R, FAST>(fy, L, A.s.my[c], A.st.sy[c], R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
+                                   A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c, A.s.om,
+                                   north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn);
         A.s.v1[c] = un;
         A.s.my1[c] = mn;
         A.s.qy1[c] = qn;
@@ -584,7 +695,7 @@ __global__ __launch_bounds__(NT) void faces_kernel(StepArgs<R> A) {
         w = m > w ? m : w;
     }
     if (want_wave) {
-        const R m = block_max(w, red);
+        const R m = block_max<NTH>(w, red);
         if (threadIdx.x == 0) A.wave[t] = m;
     }
 }
@@ -602,21 +713,35 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     __shared__ __align__(16) R s_r[LSIZE];
     __shared__ int nb[9];
     __shared__ int any_wet;
-    if ((int)blockIdx.x >= A.nact) return;
-    const int t = A.active[blockIdx.x];
+    __shared__ double red_src[NT / 64];
+    const int slot = xcd_slot(blockIdx.x, A.members ? A.glob->nact : A.nact);
+    if (slot < 0) return;
+    const int t = A.active[slot];
+    R dt_run = A.dt;
+    int want_run = A.want_wave;
+    int new_window = 0;
+    double scale = 0.0;
+    if (A.members) {
+        const Member& mb = A.members[A.tile_member[t]];
+        if (!mb.stepping) return;
+        dt_run = (R)mb.dt_step;
+        want_run = mb.refresh;
+        new_window = mb.new_window;
+        scale = mb.scale;
+    }
     load_nb3(t, A.st.nbr, nb);
     if (threadIdx.x == 0) any_wet = 0;
     __syncthreads();
     {
-        const Stager sg = make_stager(t, nb);
+        const Stager<NT> sg = make_stager<NT>(t, nb);
         R* const dst[3] = {s_h, s_qx, s_qy};
         const R* const src[3] = {A.s.h, A.s.qx1, A.s.qy1};
-        stage<R, 3>(dst, src, sg);
+        stage<R, 3, NT>(dst, src, sg);
     }
     __syncthreads();
     const R dx = R(A.P.dx);
     const R area = dx * dx;
-    const R dt = A.dt;
+    const R dt = dt_run;
     constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32
     for (int p = threadIdx.x; p < RW * RW; p += NT) {
         const int jj = p / RW, ii = p - (p / RW) * RW;
@@ -628,8 +753,9 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     }
     __syncthreads();
     const int i = threadIdx.x & (TILE - 1);
-    const bool want_plan = A.want_wave != 0;
+    const bool want_plan = want_run != 0;
     int wet = 0;
+    double src_sum = 0.0;
 #pragma unroll 1
     for (int r = 0; r < ROWS_PER_THREAD; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NT / TILE);
@@ -643,19 +769,37 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
         A.s.qx[c] = qw;
         A.s.qy[c] = qn;
         const R h0 = s_h[L];
-        const R inc = A.s.res[c] + dt * A.s.src[c] - dv<FAST>(dt * (qe - qw + qs - qn), area);
+        R src_c;
+        if (new_window) {
+            // the window's rate of this cell: float(weight * scale), the product in double, as the host forms it
+            src_c = A.src_weight ? (R)(A.src_weight[c] * scale) : R(0);
+            A.s.src[c] = src_c;
+            src_sum += (double)src_c;
+        } else {
+            src_c = A.s.src[c];
+        }
+        const R inc = A.s.res[c] + dt * src_c - dv<FAST>(dt * (qe - qw + qs - qn), area);
         const R sum = h0 + inc;
         const R hn = pmax(sum, R(0));
         A.s.res[c] = sum > R(0) ? inc - (sum - h0) : R(0);
         A.s.h_next[c] = hn;
         const R Hc = R(A.st.hc[c]);
-        A.s.d[c] = cell_level<R, FAST>(hn, Hc, A.st.ctab + c, A.st.ncell, A.st.last);
-        if (want_plan) A.s.om[c] = cell_plan<R, FAST>(hn, Hc, A.st.ctab + c, A.st.ncell, A.st.last);
+        R plan;
+        A.s.d[c] = cell_eval<R, FAST>(hn, Hc, A.st.ctab + c, A.st.ncell, A.st.last, A.st.cc + c, A.st.ctag + c,
+                                      want_plan, plan);
+        if (want_plan) A.s.om[c] = plan;
         wet |= hn > R(0);
     }
     if (wet) any_wet = 1;
+    if (new_window) {
+        for (int off = 32; off > 0; off >>= 1) src_sum += __shfl_xor(src_sum, off);
+        if ((threadIdx.x & 63) == 0) red_src[threadIdx.x >> 6] = src_sum;
+    }
     __syncthreads();
-    if (threadIdx.x == 0) A.wet[t] = any_wet;
+    if (threadIdx.x == 0) {
+        A.wet[t] = any_wet;
+        if (new_window) A.src_partial[t] = red_src[0] + red_src[1] + red_src[2] + red_src[3];
+    }
 }
 
 // Wet share of the listed tiles' cells (cells.cu:66-77).
@@ -815,6 +959,336 @@ __global__ void scale_source_kernel(size_t n, const double* __restrict__ w, doub
         src[p] = (R)(w[p] * s);
 }
 
+// ------------------------------------------------------------------------------------------------------------
+// device-resident run: the time loop of wadi2d/solver/stepper.py:101-129 and the window loop of
+// wadi2d/run.py:1712-1739, per member, with no host in the loop. A step is four launches: `pre_kernel` (one
+// workgroup: the active-list check the last step asked for, the clocks and source windows, the list change they
+// ask for, woken and leaving tiles), the face and cell kernels, and `post_kernel` (one workgroup per member: its
+// boundary cells, tally, wave-speed reduction, rain volume, clock and next step length).
+
+constexpr int SLEEP_CHECKS = 2;  // solver/stepper.py:14
+constexpr int PRE_THREADS = 1024;
+
+// Opens the member's next source window when its clock has reached the last one's end (run.py:1713-1733 with
+// model.py:112-122: new rates, kept tiles, a forced wave refresh), decides this step's refresh and length
+// (stepper.py:105-108), and retires a member that has reached its end or failed.
+__device__ void ctl_begin(Member& b, const double* __restrict__ scales, const RunNum& num, int& any) {
+    b.stepping = 0;
+    b.new_window = 0;
+    b.refresh = 0;
+    b.rebuild = 0;
+    if (b.done) return;
+    if (b.failed || b.t >= b.t_end - 1e-9) {
+        b.done = 1;
+        b.rebuild = 3;
+        any = 1;
+        return;
+    }
+    if (b.t >= b.window_end - 1e-9) {
+        const int k = b.win;
+        const double scale = k < b.nwin ? scales[b.scale_off + k] : 0.0;
+        b.rain_done += b.rain_window;
+        b.rain_window = 0.0;
+        b.window_start = b.window_end;
+        b.window_end = fmin(b.window_start + b.window_len, b.t_end);
+        b.win = k + 1;
+        if (scale != 0.0 || !b.dry) {
+            b.scale = scale;
+            b.new_window = 1;
+            b.refresh_due = 1;
+            b.rebuild = 1;
+            any = 1;
+        }
+        b.dry = !(scale > 0.0);
+    }
+    b.refresh = b.refresh_due || (b.nsteps % num.dt_every) == 0;
+    b.refresh_due = 0;
+    b.dt_step = fmin(fmin(b.dt, b.window_end - b.t), num.dt_max);
+    b.stepping = 1;
+}
+
+// Block-wide exclusive prefix of one flag per thread over a chunk of PRE_THREADS: returns this thread's slot and
+// adds the chunk's count to `base` (in every thread).
+__device__ __forceinline__ int chunk_slot(bool flag, int* wave_cnt, int& base) {
+    const unsigned long long mask = __ballot(flag);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int below = __popcll(mask & ((1ull << lane) - 1ull));
+    if (lane == 0) wave_cnt[wv] = __popcll(mask);
+    __syncthreads();
+    int off = 0, total = 0;
+    for (int q = 0; q < PRE_THREADS / 64; ++q) {
+        const int c = wave_cnt[q];
+        if (q < wv) off += c;
+        total += c;
+    }
+    __syncthreads();
+    const int slot = base + off + below;
+    base += total;
+    return slot;
+}
+
+struct RunLists {
+    const int* tile_member;
+    const int* nbr;
+    const int* wet;
+    const int* has_source;
+    const int* bnd_tile;
+    int* is_active;
+    int* idle;
+    int* flags;
+    int* active;
+    int* woken;
+    int* leaving;
+    int ntiles;
+};
+
+// The active list (stepper.py:56-92) per member, by its mode: 1 keeps what is listed and adds wet and kept tiles
+// with their neighbours (retain); 2 does that and lets a tile go after SLEEP_CHECKS checks that did not want it
+// (check); 3 drops the member's tiles (done); 0 leaves them. One workgroup, so the list stays in ascending order.
+// Woken and leaving tiles go to their lists; returns nothing, writes nact, n_woken, n_leaving.
+template <bool CHECK>
+__device__ void rebuild(const Member* __restrict__ mem, RunGlobals* __restrict__ g, const RunLists& L, int* wave_cnt) {
+    auto mode_of = [&](int t) {
+        const Member& b = mem[L.tile_member[t]];
+        return CHECK ? (b.check ? 2 : 0) : b.rebuild;
+    };
+    for (int t = threadIdx.x; t < L.ntiles; t += PRE_THREADS) {
+        int wanted = 0;
+        if (t > 0) {
+            const int mode = mode_of(t);
+            if (mode == 1 || mode == 2) {
+                const Member& b = mem[L.tile_member[t]];
+                const int keep = (L.has_source[t] && b.scale != 0.0) || L.bnd_tile[t];
+                wanted = L.wet[t] || keep;
+            }
+        }
+        L.flags[t] = wanted;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < L.ntiles; t += PRE_THREADS) {
+        const int cur = L.is_active[t];
+        int a = 0;
+        if (t > 0) {
+            const int mode = mode_of(t);
+            a = cur;
+            if (mode == 3) {
+                a = 0;
+            } else if (mode == 1 || mode == 2) {
+                a = L.flags[t] & 1;
+                for (int q = 0; q < 8; ++q) a |= L.flags[L.nbr[(size_t)t * 8 + q]] & 1;
+                if (mode == 1) {
+                    a |= cur;
+                } else {
+                    L.idle[t] = a ? 0 : L.idle[t] + 1;
+                    a |= cur && L.idle[t] < SLEEP_CHECKS;
+                }
+            }
+        }
+        L.flags[t] |= a << 1;
+    }
+    __syncthreads();
+    int na = 0, nw = 0, nl = 0;
+    for (int base = 0; base < L.ntiles; base += PRE_THREADS) {
+        const int t = base + threadIdx.x;
+        const bool in = t < L.ntiles;
+        const int a = in ? (L.flags[t] >> 1) & 1 : 0;
+        const int cur = in ? L.is_active[t] : 0;
+        const int mode = (in && t > 0) ? mode_of(t) : 0;
+        const bool wk = a && !cur && (mode == 1 || mode == 2);
+        const bool lv = cur && !a;
+        const int sa = chunk_slot(a, wave_cnt, na);
+        const int sw = chunk_slot(wk, wave_cnt, nw);
+        const int sl = chunk_slot(lv, wave_cnt, nl);
+        if (a) L.active[sa] = t;
+        if (wk) L.woken[sw] = t;
+        if (lv) L.leaving[sl] = t;
+        if (in) L.is_active[t] = a;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        g->nact = na;
+        g->n_woken = nw;
+        g->n_leaving = nl;
+    }
+    __syncthreads();
+}
+
+// What a change of the list does to tiles (stepper.py:94-99): a woken tile's wet share from its depth, by the
+// kernel's own arithmetic; a tile that leaves keeps one mean depth in both buffers, and a finished member's tile
+// also its face buffers.
+template <class R, bool FAST>
+__device__ void transitions(const RunGlobals* __restrict__ g, const Member* __restrict__ mem, const RunLists& L,
+                            R* __restrict__ h, R* __restrict__ h_other, R* __restrict__ om,
+                            R* const* __restrict__ faces_cur, R* const* __restrict__ faces_other,
+                            const float* __restrict__ hc, const float2* __restrict__ ctab, size_t ncell, int last) {
+    const int nw = g->n_woken, nl = g->n_leaving;
+    for (int b = 0; b < nw; ++b) {
+        const size_t c = (size_t)L.woken[b] * TILE2 + threadIdx.x;
+        om[c] = cell_plan<R, FAST>(h[c], R(hc[c]), ctab + c, ncell, last);
+    }
+    for (int b = 0; b < nl; ++b) {
+        const int t = L.leaving[b];
+        const size_t c = (size_t)t * TILE2 + threadIdx.x;
+        h_other[c] = h[c];
+        if (mem[L.tile_member[t]].done)
+            for (int f = 0; f < 4; ++f) faces_other[f][c] = faces_cur[f][c];
+    }
+    __syncthreads();
+}
+
+// Before a step, one workgroup: the check the last step asked for (stepper.py:113-114), the clocks and windows,
+// the list change those ask for, and the tiles that woke or left.
+template <class R, bool FAST>
+__global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ mem, int M, RunGlobals* __restrict__ g,
+                                                          const double* __restrict__ scales, const RunNum num,
+                                                          RunLists L, R* __restrict__ h, R* __restrict__ h_other,
+                                                          R* __restrict__ om, R* const* __restrict__ faces_cur,
+                                                          R* const* __restrict__ faces_other,
+                                                          const float* __restrict__ hc,
+                                                          const float2* __restrict__ ctab, size_t ncell, int last,
+                                                          int flush_only) {
+    static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
+    __shared__ int wave_cnt[PRE_THREADS / 64];
+    __shared__ int any_check, any;
+    if (threadIdx.x == 0) {
+        any_check = 0;
+        any = 0;
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < M; m += PRE_THREADS)
+        if (mem[m].check) any_check = 1;
+    __syncthreads();
+    if (any_check) {
+        rebuild<true>(mem, g, L, wave_cnt);
+        transitions<R, FAST>(g, mem, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+        for (int m = threadIdx.x; m < M; m += PRE_THREADS) mem[m].check = 0;
+        __syncthreads();
+    }
+    if (flush_only) return;
+    int mine = 0;
+    for (int m = threadIdx.x; m < M; m += PRE_THREADS) ctl_begin(mem[m], scales, num, mine);
+    if (mine) any = 1;
+    __syncthreads();
+    if (any) {
+        rebuild<false>(mem, g, L, wave_cnt);
+        transitions<R, FAST>(g, mem, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+    }
+}
+
+// After a step, one workgroup per member: its boundary cells (cells.cu:95-111) and FP64 tally
+// (engine/maxima.py:53-62), the wave-speed reduction and the next step length with the failure rules
+// (stepper.py:109-126), the window's rain volume, the clock, and the check every active_every steps.
+template <class R>
+__global__ __launch_bounds__(256) void post_kernel(Member* __restrict__ mem, const R* __restrict__ wave,
+                                                   const int* __restrict__ is_active,
+                                                   const double* __restrict__ src_partial, const RunNum num,
+                                                   const int* __restrict__ cell, const int* __restrict__ ref,
+                                                   const int* __restrict__ kind, const float* __restrict__ d0,
+                                                   const float* __restrict__ h0, R* __restrict__ h, R* __restrict__ d,
+                                                   const int* __restrict__ face, const int* __restrict__ axis,
+                                                   const float* __restrict__ sign, const R* __restrict__ qx,
+                                                   const R* __restrict__ qy, double* __restrict__ total,
+                                                   double* __restrict__ flow) {
+    __shared__ double red_w[4], red_s[4];
+    Member& b = mem[blockIdx.x];
+    if (!b.stepping) return;
+    for (int p = b.b0 + threadIdx.x; p < b.b1; p += blockDim.x) {
+        const int k = cell[p];
+        if (kind[p] == 0) {
+            h[k] = R(h0[p]);
+            d[k] = R(d0[p]);
+        } else if (kind[p] == 1) {
+            h[k] = R(0);
+            d[k] = R(0);
+        } else {
+            h[k] = h[ref[p]];
+            d[k] = pmax(d[ref[p]] + R(d0[p]), R(0));
+        }
+    }
+    for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) {
+        const R qq = axis[q] ? qy[face[q]] : qx[face[q]];
+        const double f = (double)(R(sign[q]) * qq);
+        total[q] += b.dt_step * f;
+        flow[q] = f;
+    }
+    double w = 0.0, rs = 0.0;
+    if (b.refresh || b.new_window) {
+        for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x) {
+            if (!is_active[t]) continue;
+            if (b.refresh) {
+                const double v = (double)wave[t];
+                w = v > w ? v : w;
+            }
+            if (b.new_window) rs += src_partial[t];
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(w, off);
+        w = o > w ? o : w;
+        rs += __shfl_xor(rs, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red_w[threadIdx.x >> 6] = w;
+        red_s[threadIdx.x >> 6] = rs;
+    }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    w = red_w[0];
+    rs = red_s[0];
+    for (int q = 1; q < 4; ++q) {
+        w = red_w[q] > w ? red_w[q] : w;
+        rs += red_s[q];
+    }
+    if (b.new_window) b.rain_window = rs * num.area * (b.window_end - b.window_start);
+    const long long before = b.nsteps;
+    b.t += b.dt_step;
+    b.nsteps = before + 1;
+    if (b.refresh) {
+        if (!isfinite(w)) {
+            b.failed = 1;
+        } else {
+            b.dt = fmin(num.cfl * num.dx / fmax(w, num.wave_floor), num.dt_max);
+            if (b.dt < num.dt_min) b.failed = 2;
+        }
+    }
+    if (before % num.active_every == 0) b.check = 1;
+}
+
+// Running maxima after the step, at the member's new time (engine/maxima.py:29-51).
+template <class R>
+__global__ __launch_bounds__(NT) void maxima_run_kernel(const RunGlobals* __restrict__ g, const Member* __restrict__ mem,
+                                                        const int* __restrict__ tile_member,
+                                                        const int* __restrict__ active, const int* __restrict__ nbr,
+                                                        R arrive, R deep, const R* __restrict__ d,
+                                                        const R* __restrict__ h, const R* __restrict__ u,
+                                                        const R* __restrict__ v, R* __restrict__ dmax,
+                                                        R* __restrict__ hmax, R* __restrict__ smax,
+                                                        R* __restrict__ zmax, R* __restrict__ tarr) {
+    if ((int)blockIdx.x >= g->nact) return;
+    const int t = active[blockIdx.x];
+    const Member& b = mem[tile_member[t]];
+    if (!b.stepping) return;
+    const R time = (R)b.t;
+    const int te = nbr[(size_t)t * 8 + 4], ts = nbr[(size_t)t * 8 + 6];
+    for (int p = threadIdx.x; p < TILE2; p += NT) {
+        const int j = p / TILE, i = p - (p / TILE) * TILE;
+        const size_t k = (size_t)t * TILE2 + p;
+        const R dk = d[k];
+        if (dk > dmax[k]) dmax[k] = dk;
+        if (h[k] > hmax[k]) hmax[k] = h[k];
+        if (dk > deep) {
+            const size_t ke = i < TILE - 1 ? k + 1 : (size_t)te * TILE2 + j * TILE;
+            const size_t ks = j < TILE - 1 ? k + TILE : (size_t)ts * TILE2 + i;
+            const R uc = R(0.5) * (u[k] + u[ke]);
+            const R vc = R(0.5) * (v[k] + v[ks]);
+            const R sp = sqrt(uc * uc + vc * vc);
+            if (sp > smax[k]) smax[k] = sp;
+            if (sp * dk > zmax[k]) zmax[k] = sp * dk;
+        }
+        if (tarr[k] < R(0) && dk >= arrive) tarr[k] = time;
+    }
+}
+
 // A plain copy, the yardstick for the memory counters: n float4 read once and written once.
 __global__ __launch_bounds__(256) void copy_probe_kernel(size_t n, const float4* __restrict__ a, float4* __restrict__ b) {
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) b[p] = a[p];
@@ -833,6 +1307,8 @@ __global__ void fill_kernel(R* a, size_t n, R v) {
 struct w2d_handle {
     int prec = 4;
     int fast = 0;
+    int face_cache = 1;
+    int faces_threads = 512;
     int ntiles = 0;
     int rows = 0;
     size_t ncell = 0;
@@ -852,6 +1328,9 @@ struct w2d_handle {
     float2* ctab = nullptr;
     float4 *ftabx = nullptr, *ftaby = nullptr, *sx = nullptr, *sy = nullptr, *ec = nullptr;
     float *zcx = nullptr, *zcy = nullptr;
+    float4 *fcx = nullptr, *fcy = nullptr;
+    float2* cc = nullptr;
+    unsigned char *ftagx = nullptr, *ftagy = nullptr, *ctag = nullptr;
     int* nbr = nullptr;
     std::vector<int32_t> nbr_host;
     // lists
@@ -873,6 +1352,21 @@ struct w2d_handle {
     double* scalar = nullptr;  // reduction results
     double* src_weight = nullptr;
     double* partial = nullptr;  // SUM_BLOCKS doubles
+    // device-resident run
+    int run_m = 0;
+    Member* run_mem = nullptr;
+    RunGlobals* run_glob = nullptr;
+    RunNum run_num{};
+    int *run_tile_member = nullptr, *run_has_source = nullptr, *run_bnd_tile = nullptr, *run_is_active = nullptr,
+        *run_idle = nullptr, *run_flags = nullptr, *run_woken = nullptr, *run_leaving = nullptr;
+    double *run_scales = nullptr, *run_src_partial = nullptr;
+    void** run_faces_ptrs = nullptr;  // [parity][cur 4, other 4]
+    int run_maxima = 0;
+    std::vector<double> weight_host;
+    hipGraphExec_t graph = nullptr;
+    int graph_steps = 0;
+    int graph_parity = 0;  // the buffer parity the graph was captured at; it replays only at that parity
+
     // kernel timing: when on, every step records events around its two kernels on the handle's stream
     int timing = 0;
     std::vector<hipEvent_t> event_pool;
@@ -956,6 +1450,12 @@ StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     A.st.ec = H->ec;
     A.st.zcx = H->zcx;
     A.st.zcy = H->zcy;
+    A.st.fcx = H->face_cache ? H->fcx : nullptr;
+    A.st.fcy = H->face_cache ? H->fcy : nullptr;
+    A.st.ftagx = H->ftagx;
+    A.st.ftagy = H->ftagy;
+    A.st.cc = H->cc;
+    A.st.ctag = H->ctag;
     A.st.nbr = H->nbr;
     A.st.ncell = H->ncell;
     A.st.last = H->rows - 1;
@@ -983,13 +1483,24 @@ int mark(w2d_handle* H) {
     return 0;
 }
 
+// The face kernel at the handle's workgroup size: 256 threads (4 cells each) or 512 (2 cells each; the same LDS
+// per workgroup, twice the waves per CU).
+template <class R, bool FAST>
+int launch_faces(w2d_handle* H, const StepArgs<R>& A, int blocks) {
+    if (H->faces_threads == 512)
+        faces_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
+    else
+        faces_kernel<R, FAST, 256><<<blocks, 256, 0, H->stream>>>(A);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
 template <class R, bool FAST>
 int launch_step(w2d_handle* H, double dt, int want_wave) {
     const StepArgs<R> A = step_args<R>(H, dt, want_wave);
     int s;
     if (H->timing && (s = mark(H))) return s;
-    faces_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
-    HIP_TRY(hipGetLastError());
+    if ((s = launch_faces<R, FAST>(H, A, H->nact))) return s;
     if (H->timing && (s = mark(H))) return s;
     cells_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
     HIP_TRY(hipGetLastError());
@@ -1079,7 +1590,8 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     auto* H = new w2d_handle();
     *out = nullptr;
     H->prec = precision;
-    H->fast = precision == W2D_F32 && variant == W2D_FAST;
+    H->fast = precision == W2D_F32 && (variant & 1) == W2D_FAST;
+    H->face_cache = (variant & W2D_NO_FACE_CACHE) == 0;
     H->ntiles = T->ntiles;
     H->rows = T->rows;
     H->ncell = (size_t)T->ntiles * TILE2;
@@ -1144,14 +1656,17 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     H->nbr_host.assign(T->nbr, T->nbr + (size_t)H->ntiles * 8);
     for (int q = 0; q < 8; ++q) H->nbr_host[q] = 0;  // the void tile's neighbours are the void
     if ((s = upload(H, &H->nbr, H->nbr_host.data(), H->nbr_host.size()))) return bail(s);
-    {
-        // centre-line beds with their halos filled, as static_cell_state (state.py:216-223)
-        std::vector<float> zx((size_t)H->ntiles * STRIDE2), zy((size_t)H->ntiles * STRIDE2);
-        expand(H, T->zcx, zx.data());
-        expand(H, T->zcy, zy.data());
-        for (size_t q = 0; q < (size_t)STRIDE2; ++q) zx[q] = zy[q] = 0.0f;
-        if ((s = upload(H, &H->zcx, zx.data(), zx.size()))) return bail(s);
-        if ((s = upload(H, &H->zcy, zy.data(), zy.size()))) return bail(s);
+    // centre-line beds in the own layout; a face reads its upwind neighbour's from that tile (the void's is 0),
+    // which is what static_cell_state's filled halos hold (state.py:216-223)
+    if ((s = upload(H, &H->zcx, T->zcx, n))) return bail(s);
+    if ((s = upload(H, &H->zcy, T->zcy, n))) return bail(s);
+    // row caches, empty
+    if ((s = dalloc(H, (void**)&H->fcx, n * sizeof(float4)))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->fcy, n * sizeof(float4)))) return bail(s);
+    if ((s = dalloc(H, (void**)&H->cc, n * sizeof(float2)))) return bail(s);
+    for (unsigned char** tg : {&H->ftagx, &H->ftagy, &H->ctag}) {
+        if ((s = dalloc(H, (void**)tg, n))) return bail(s);
+        if (hipMemsetAsync(*tg, 0xFF, n, H->stream) != hipSuccess) return bail(fail(3, "create: tag fill"));
     }
     std::vector<int32_t> all;
     for (int t = 1; t < H->ntiles; ++t) all.push_back(t);
@@ -1166,6 +1681,7 @@ int w2d_destroy(w2d_handle* H) {
     if (H->stream) (void)hipStreamSynchronize(H->stream);
     for (hipEvent_t e : H->marks) (void)hipEventDestroy(e);
     for (hipEvent_t e : H->event_pool) (void)hipEventDestroy(e);
+    if (H->graph) (void)hipGraphExecDestroy(H->graph);
     for (void* p : H->owned) (void)hipFree(p);
     if (H->stream) (void)hipStreamDestroy(H->stream);
     delete H;
@@ -1502,6 +2018,7 @@ int w2d_source_weights(w2d_handle* H, const int64_t* index, const double* w, int
         if (c < 0) return fail(1, "source_weights: an index is not an own cell");
         own[c] = w[m];
     }
+    H->weight_host = own;
     if (!H->src_weight) {
         int s = dalloc(H, (void**)&H->src_weight, H->ncell * sizeof(double));
         if (s) return s;
@@ -1533,8 +2050,14 @@ int w2d_timing(w2d_handle* H, int32_t on) {
     return 0;
 }
 
+int w2d_timing_read_marks(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches);
+
 int w2d_timing_read(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches) {
     if (!H || !faces_ms || !cells_ms || !launches) return fail(1, "timing_read: bad arguments");
+    return w2d_timing_read_marks(H, faces_ms, cells_ms, launches);
+}
+
+int w2d_timing_read_marks(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches) {
     HIP_TRY(hipStreamSynchronize(H->stream));
     double f = 0.0, c = 0.0;
     const size_t n = H->marks.size() / 3;
@@ -1581,9 +2104,308 @@ int w2d_copy_probe(int64_t bytes, int32_t repeats, double* ms) {
     return 0;
 }
 
+int w2d_set_launch(w2d_handle* H, int32_t faces_threads) {
+    if (!H) return fail(1, "set_launch: no handle");
+    if (faces_threads != 256 && faces_threads != 512) return fail(1, "set_launch: faces_threads is 256 or 512");
+    if (faces_threads != H->faces_threads && H->graph) {  // a captured graph holds the old launches
+        (void)hipGraphExecDestroy(H->graph);
+        H->graph = nullptr;
+        H->graph_steps = 0;
+    }
+    H->faces_threads = faces_threads;
+    return 0;
+}
+
 int w2d_synchronize(w2d_handle* H) {
     if (!H) return fail(1, "synchronize: no handle");
     HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+}  // extern "C"
+
+// ---- device-resident run: host side
+
+
+namespace {
+
+template <class R>
+RunLists run_lists(w2d_handle* H) {
+    return RunLists{H->run_tile_member, H->nbr, H->wet, H->run_has_source, H->run_bnd_tile, H->run_is_active,
+                    H->run_idle, H->run_flags, H->active, H->run_woken, H->run_leaving, H->ntiles};
+}
+
+template <class R, bool FAST>
+int enqueue_pre(w2d_handle* H, int flush_only) {
+    R* const* fc = (R* const*)(H->run_faces_ptrs + 8 * H->ucur);  // current u, v, mx, my, then the others
+    pre_kernel<R, FAST><<<1, PRE_THREADS, 0, H->stream>>>(
+        H->run_mem, H->run_m, H->run_glob, H->run_scales, H->run_num, run_lists<R>(H), (R*)H->h[H->hcur],
+        (R*)H->h[1 - H->hcur], (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, flush_only);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <class R, bool FAST>
+int enqueue_run_step(w2d_handle* H) {
+    StepArgs<R> A = step_args<R>(H, 0.0, 0);
+    A.members = H->run_mem;
+    A.tile_member = H->run_tile_member;
+    A.glob = H->run_glob;
+    A.src_weight = H->src_weight;
+    A.src_partial = H->run_src_partial;
+    const int nt = H->ntiles;
+    R* h_new = (R*)H->h[1 - H->hcur];
+    int s;
+    if ((s = enqueue_pre<R, FAST>(H, 0))) return s;
+    if (H->timing && (s = mark(H))) return s;
+    if ((s = launch_faces<R, FAST>(H, A, nt))) return s;
+    if (H->timing && (s = mark(H))) return s;
+    cells_kernel<R, FAST><<<nt, NT, 0, H->stream>>>(A);
+    if (H->timing && (s = mark(H))) return s;
+    post_kernel<R><<<H->run_m, 256, 0, H->stream>>>(H->run_mem, (const R*)H->wave, H->run_is_active,
+                                                     H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
+                                                     H->b_d0, H->b_h0, h_new, (R*)H->d, H->t_face, H->t_axis,
+                                                     H->t_sign, (R*)H->qx, (R*)H->qy, H->t_total, H->t_flow);
+    if (H->run_maxima)
+        maxima_run_kernel<R><<<nt, NT, 0, H->stream>>>(
+            H->run_glob, H->run_mem, H->run_tile_member, H->active, H->nbr, (R)H->m_arrive, (R)H->m_deep, (R*)H->d,
+            h_new, (R*)H->uv[0][1 - H->ucur], (R*)H->uv[1][1 - H->ucur], (R*)H->maxima[0], (R*)H->maxima[1],
+            (R*)H->maxima[2], (R*)H->maxima[3], (R*)H->maxima[4]);
+    HIP_TRY(hipGetLastError());
+    H->hcur = 1 - H->hcur;
+    H->ucur = 1 - H->ucur;
+    return 0;
+}
+
+int run_flush(w2d_handle* H) {
+    if (H->prec == 8) return enqueue_pre<double, false>(H, 1);
+    if (H->fast) return enqueue_pre<float, true>(H, 1);
+    return enqueue_pre<float, false>(H, 1);
+}
+
+int run_step_any(w2d_handle* H) {
+    if (H->prec == 8) return enqueue_run_step<double, false>(H);
+    if (H->fast) return enqueue_run_step<float, true>(H);
+    return enqueue_run_step<float, false>(H);
+}
+
+}  // namespace
+
+extern "C" {
+
+int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const double* scales, int64_t nscales,
+                  const w2d_run_numerics* num, const int32_t* idle) {
+    if (!H || !members || M < 1 || !num || (nscales && !scales) || num->dt_every < 1 || num->active_every < 1)
+        return fail(1, "run_setup: bad arguments");
+    const int nt = H->ntiles;
+    int s;
+    std::vector<int32_t> tm(nt, 0);
+    std::vector<Member> mem(M);
+    for (int m = 0; m < M; ++m) {
+        const w2d_member& d = members[m];
+        if (d.tile0 < 1 || d.tile1 > nt || d.tile0 >= d.tile1 || d.window <= 0.0 || d.nwindows < 0 ||
+            d.scale_offset < 0 || d.scale_offset + d.nwindows > nscales)
+            return fail(1, "run_setup: a member's tiles or windows are out of range");
+        for (int t = d.tile0; t < d.tile1; ++t) {
+            if (tm[t] || (m == 0 && t == 0)) return fail(1, "run_setup: members overlap");
+            tm[t] = m;
+        }
+        Member& b = mem[m];
+        std::memset(&b, 0, sizeof(b));
+        b.t = d.t0;
+        b.t_end = d.t_end;
+        b.dt = d.dt0;
+        b.window_start = d.t0;
+        b.window_end = d.t0;
+        b.window_len = d.window;
+        b.nsteps = d.nsteps0;
+        b.scale_off = d.scale_offset;
+        b.nwin = d.nwindows;
+        b.tile0 = d.tile0;
+        b.tile1 = d.tile1;
+        b.refresh_due = d.refresh_due;
+    }
+    // tiles that keep themselves active: a source, a boundary cell (model.py:117-122, 145-150); each member's
+    // boundary cells and tally faces must be one run of the lists, in member order
+    std::vector<int32_t> has_source(nt, 0), bnd_tile(nt, 0), is_active(nt, 0);
+    if (!H->weight_host.empty())
+        for (size_t c = 0; c < H->ncell; ++c)
+            if (H->weight_host[c] != 0.0) has_source[c / TILE2] = 1;
+    auto ranges = [&](const int* dev, int n, bool cells, const char* what) -> int {
+        std::vector<int32_t> v(std::max(n, 1));
+        if (n) HIP_TRY(hipMemcpy(v.data(), dev, n * sizeof(int), hipMemcpyDeviceToHost));
+        int q = 0;
+        for (int m = 0; m < M; ++m) {
+            const int first = q;
+            while (q < n && tm[v[q] / TILE2] == m && v[q] / TILE2 >= members[m].tile0 && v[q] / TILE2 < members[m].tile1) {
+                if (cells) bnd_tile[v[q] / TILE2] = 1;
+                ++q;
+            }
+            if (cells) {
+                mem[m].b0 = first;
+                mem[m].b1 = q;
+            } else {
+                mem[m].f0 = first;
+                mem[m].f1 = q;
+            }
+        }
+        if (q != n) return fail(1, std::string("run_setup: the ") + what + " are not grouped by member in member order");
+        return 0;
+    };
+    if ((s = ranges(H->b_cell, H->nb_cells, true, "boundary cells"))) return s;
+    if ((s = ranges(H->t_face, H->n_tally, false, "tally faces"))) return s;
+    for (int t : H->active_host) is_active[t] = 1;
+    std::vector<int32_t> idle_v(nt, 0);
+    if (idle) idle_v.assign(idle, idle + nt);
+    H->run_m = 0;
+    auto up = [&](int** dst, const std::vector<int32_t>& v) -> int {
+        if (*dst) {
+            HIP_TRY(hipMemcpy(*dst, v.data(), std::max<size_t>(v.size(), 1) * sizeof(int32_t), hipMemcpyHostToDevice));
+            return 0;
+        }
+        return upload(H, dst, v.data(), std::max<size_t>(v.size(), 1));
+    };
+    std::vector<int32_t> zeros(nt, 0);
+    if ((s = up(&H->run_tile_member, tm)) || (s = up(&H->run_has_source, has_source)) ||
+        (s = up(&H->run_bnd_tile, bnd_tile)) || (s = up(&H->run_is_active, is_active)) ||
+        (s = up(&H->run_idle, idle_v)) || (s = up(&H->run_flags, zeros)) || (s = up(&H->run_woken, zeros)) ||
+        (s = up(&H->run_leaving, zeros)))
+        return s;
+    if (H->run_mem) {
+        H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_mem), H->owned.end());
+        (void)hipFree(H->run_mem);
+        H->run_mem = nullptr;
+    }
+    if ((s = upload(H, &H->run_mem, mem.data(), M))) return s;
+    if (H->run_scales) {
+        H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_scales), H->owned.end());
+        (void)hipFree(H->run_scales);
+        H->run_scales = nullptr;
+    }
+    std::vector<double> sc(scales, scales + nscales);
+    if (sc.empty()) sc.push_back(0.0);
+    if ((s = upload(H, &H->run_scales, sc.data(), sc.size()))) return s;
+    if (!H->run_glob) {
+        RunGlobals g0{};
+        if ((s = upload(H, &H->run_glob, &g0, 1))) return s;
+    }
+    RunGlobals g{};
+    g.nact = H->nact;
+    HIP_TRY(hipMemcpy(H->run_glob, &g, sizeof(g), hipMemcpyHostToDevice));
+    if (!H->run_src_partial && (s = dalloc(H, (void**)&H->run_src_partial, (size_t)nt * sizeof(double)))) return s;
+    if (!H->run_faces_ptrs) {
+        // [ucur = 0: current u, v, mx, my, then the others][ucur = 1: the same, swapped]
+        void* ptrs[16];
+        for (int par = 0; par < 2; ++par)
+            for (int f = 0; f < 4; ++f) {
+                ptrs[8 * par + f] = H->uv[f][par];
+                ptrs[8 * par + 4 + f] = H->uv[f][1 - par];
+            }
+        if ((s = upload(H, (void***)&H->run_faces_ptrs, (void**)ptrs, 16))) return s;
+    }
+    H->run_num = RunNum{num->cfl, num->dt_max, num->dt_min, num->wave_floor, H->P.dx, H->P.dx * H->P.dx,
+                        num->dt_every, num->active_every};
+    H->run_maxima = num->maxima != 0;
+    if (H->run_maxima && !H->maxima[0]) return fail(1, "run_setup: maxima asked for but maxima_set not called");
+    if (H->graph) {
+        (void)hipGraphExecDestroy(H->graph);
+        H->graph = nullptr;
+        H->graph_steps = 0;
+    }
+    H->run_m = M;
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
+    if (!H || !H->run_m || nsteps < 0) return fail(1, "run_steps: call run_setup first");
+    if (graph_steps > 0 && graph_steps % 2) return fail(1, "run_steps: a graph holds an even number of steps");
+    int done = 0, s;
+    // HIP cannot time events recorded inside a captured graph: timed steps are launched one by one
+    if (graph_steps > 0 && !H->timing) {
+        if (H->graph && H->graph_steps != graph_steps) {
+            (void)hipGraphExecDestroy(H->graph);
+            H->graph = nullptr;
+        }
+        if (!H->graph && nsteps >= graph_steps) {
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
+            for (int q = 0; q < graph_steps; ++q)
+                if ((s = run_step_any(H))) {
+                    (void)hipStreamEndCapture(H->stream, &g);
+                    return s;
+                }
+            HIP_TRY(hipStreamEndCapture(H->stream, &g));
+            HIP_TRY(hipGraphInstantiate(&H->graph, g, nullptr, nullptr, 0));
+            (void)hipGraphDestroy(g);
+            H->graph_steps = graph_steps;
+            H->graph_parity = H->hcur;
+            // capturing enqueued nothing; the parities advanced graph_steps (even) times and are back
+        }
+        if (H->graph && H->hcur != H->graph_parity && done < nsteps) {
+            if ((s = run_step_any(H))) return s;
+            ++done;
+        }
+        while (H->graph && H->hcur == H->graph_parity && nsteps - done >= graph_steps) {
+            HIP_TRY(hipGraphLaunch(H->graph, H->stream));
+            done += graph_steps;
+        }
+    }
+    for (; done < nsteps; ++done)
+        if ((s = run_step_any(H))) return s;
+    return 0;
+}
+
+int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
+    if (!H || !out || M != H->run_m) return fail(1, "run_read: bad arguments");
+    std::vector<Member> mem(M);
+    RunGlobals g;
+    int s = run_flush(H);  // a check the last step asked for is applied now, as the host loop does after its step
+    if (s) return s;
+    HIP_TRY(hipMemcpyAsync(mem.data(), H->run_mem, M * sizeof(Member), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipMemcpyAsync(&g, H->run_glob, sizeof(g), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    for (int m = 0; m < M; ++m) {
+        const Member& b = mem[m];
+        w2d_member_state& o = out[m];
+        o.t = b.t;
+        o.dt = b.dt;
+        o.nsteps = b.nsteps;
+        o.done = b.done;
+        o.failed = b.failed;
+        o.refresh_due = b.refresh_due;
+        // rain credited pro rata in the current window (monitor.py:197-212)
+        const double len = b.window_end - b.window_start;
+        const double share = len > 0.0 ? (std::min(b.t, b.window_end) - b.window_start) / len : 0.0;
+        o.rain = b.rain_done + b.rain_window * share;
+    }
+    // the protocol's view of the list follows the device's
+    H->nact = g.nact;
+    H->active_host.resize(g.nact);
+    if (g.nact)
+        HIP_TRY(hipMemcpy(H->active_host.data(), H->active, g.nact * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int w2d_tally_totals(w2d_handle* H, double* out) {
+    if (!H || !out) return fail(1, "tally_totals: bad arguments");
+    if (H->n_tally) {
+        HIP_TRY(hipMemcpyAsync(out, H->t_total, H->n_tally * sizeof(double), hipMemcpyDeviceToHost, H->stream));
+        HIP_TRY(hipStreamSynchronize(H->stream));
+    }
+    return 0;
+}
+
+int w2d_active_tiles(w2d_handle* H, int32_t* out, int32_t* n) {
+    if (!H || !n) return fail(1, "active_tiles: bad arguments");
+    *n = H->nact;
+    if (out && H->nact) std::copy(H->active_host.begin(), H->active_host.end(), out);
+    return 0;
+}
+
+int w2d_run_idle(w2d_handle* H, int32_t* out) {
+    if (!H || !out || !H->run_idle) return fail(1, "run_idle: bad arguments");
+    HIP_TRY(hipMemcpy(out, H->run_idle, H->ntiles * sizeof(int), hipMemcpyDeviceToHost));
     return 0;
 }
 

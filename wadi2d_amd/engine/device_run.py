@@ -1,0 +1,90 @@
+"""A run with its time loop on the device: the stepper (solver/stepper.py:101-129) and the window loop
+(run.py:1712-1739) of one model, or of an ensemble of models side by side in one set of arrays, advanced by
+captured graphs of steps with no host in the loop.
+
+Each member keeps the reference's sequence: a window opens every ``source_every_s`` (new rates, kept tiles, a
+forced wave refresh), a step is ``min(dt, window end - t, dt_max)``, the wave speed is reduced every ``dt_every``
+steps into the next ``dt``, the active list is checked every ``active_every`` steps. The window's source rates are
+its precomputed scale times each cell's weight, float(weight * scale), as ``Rain.rate`` forms them on the host.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from wadi2d_amd.engine.driver import Balance
+from wadi2d_amd.engine.forcing import BoundarySet, Rain
+from wadi2d_amd.solver.hip_backend import Member, RunNumerics
+
+
+@dataclass
+class MemberPlan:
+    """One member's share of a device run."""
+
+    tile0: int
+    tile1: int
+    rain: Rain
+    boundary: BoundarySet
+    t0: float
+    t_end: float
+    dt0: float
+    nsteps0: int = 0
+    refresh_due: bool = True
+    storage_index: np.ndarray | None = None
+
+
+def window_scales(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarray:
+    """The scale of every window of [t0, t_end], as the run loop opens them."""
+    out = []
+    t = t0
+    while t < t_end - 1e-9:
+        t_next = min(t + every, t_end)
+        out.append(rain.scale(t, t_next))
+        t = t_next
+    return np.asarray(out, np.float64)
+
+
+class DeviceRun:
+    def __init__(self, backend, plans: list[MemberPlan], num, source_every_s: float = 30.0, maxima: bool = False,
+                 idle: np.ndarray | None = None) -> None:  # fmt: skip
+        self.b = backend
+        self.plans = plans
+        self.area = backend.grid.dx ** 2
+        scales, members = [], []
+        off = 0
+        for p in plans:
+            sc = window_scales(p.rain, p.t0, p.t_end, source_every_s)
+            members.append(Member(p.tile0, p.tile1, p.t0, p.t_end, p.dt0, source_every_s, sc.size, off, p.nsteps0,
+                                  int(p.refresh_due)))  # fmt: skip
+            scales.append(sc)
+            off += sc.size
+        self.scales = np.concatenate(scales) if scales else np.zeros(0)
+        rn = RunNumerics(num.cfl, num.dt_max, num.dt_min, num.wave_floor, num.dt_every, num.active_every, int(maxima))
+        backend.run_setup(members, self.scales, rn, idle)
+        self.state = backend.run_read()
+        self.storage0 = [self._storage(p) for p in plans]
+
+    def _storage(self, p: MemberPlan) -> float:
+        if p.storage_index is None:
+            return 0.0
+        self.b.storage_set(p.storage_index)
+        return self.b.storage_sum() * self.area
+
+    def steps(self, n: int, graph_steps: int = 64) -> list:
+        self.b.run_steps(n, graph_steps)
+        self.state = self.b.run_read()
+        return self.state
+
+    def run(self, chunk: int = 256, graph_steps: int = 64, on_chunk=None) -> list:
+        """Step until every member is done (or failed)."""
+        while not all(s.done for s in self.state):
+            self.steps(chunk, graph_steps)
+            if on_chunk is not None:
+                on_chunk(self.state)
+        return self.state
+
+    def balance(self, m: int, outflow: float) -> Balance:
+        p, s = self.plans[m], self.state[m]
+        return Balance(rain=s.rain, outflow=outflow, storage0=self.storage0[m], storage=self._storage(p))

@@ -63,7 +63,7 @@ class BackendUnavailable(RuntimeError):
     """No device, or the HIP library is not built."""
 
 
-def select_backend(name: str, tables, num: NumericsRecord):
+def select_backend(name: str, tables, num: NumericsRecord, **options):
     """``auto``, ``gpu`` and ``hip`` take the HIP step; ``hip-exact`` and ``hip-f64`` its other arithmetic. There is no
     silent CPU fallback: the CPU backend is the reference's own (solver/cpu), so a run without a device fails here."""
     from wadi2d_amd.solver.hip_backend import HipBackend, device_count
@@ -74,4 +74,4 @@ def select_backend(name: str, tables, num: NumericsRecord):
     if device_count() == 0:
         raise BackendUnavailable("no HIP device available")
     precision, variant = BACKENDS[key]
-    return HipBackend(tables, num, precision, variant)
+    return HipBackend(tables, num, precision, variant, **options)

@@ -36,6 +36,21 @@ class _Params(ctypes.Structure):
     ]
 
 
+class Member(ctypes.Structure):
+    _fields_ = [("tile0", I32), ("tile1", I32), ("t0", D), ("t_end", D), ("dt0", D), ("window", D),
+                ("nwindows", I32), ("scale_offset", I64), ("nsteps0", I64), ("refresh_due", I32)]  # fmt: skip
+
+
+class RunNumerics(ctypes.Structure):
+    _fields_ = [("cfl", D), ("dt_max", D), ("dt_min", D), ("wave_floor", D), ("dt_every", I32),
+                ("active_every", I32), ("maxima", I32)]  # fmt: skip
+
+
+class MemberState(ctypes.Structure):
+    _fields_ = [("t", D), ("dt", D), ("rain", D), ("nsteps", I64), ("done", I32), ("failed", I32),
+                ("refresh_due", I32)]  # fmt: skip
+
+
 _SIGS = {
     "w2d_create": [ctypes.POINTER(_Tables), ctypes.POINTER(_Params), I32, I32, ctypes.POINTER(VP)],
     "w2d_destroy": [VP],
@@ -61,11 +76,18 @@ _SIGS = {
     "w2d_storage_set": [VP, VP, I64],
     "w2d_storage": [VP, ctypes.POINTER(D)],
     "w2d_synchronize": [VP],
+    "w2d_set_launch": [VP, I32],
     "w2d_source_weights": [VP, VP, VP, I64],
     "w2d_source_scale": [VP, D],
     "w2d_source_sum": [VP, ctypes.POINTER(D)],
     "w2d_device_count": [ctypes.POINTER(I32)],
     "w2d_timing": [VP, I32],
+    "w2d_run_setup": [VP, ctypes.POINTER(Member), I32, VP, I64, ctypes.POINTER(RunNumerics), VP],
+    "w2d_run_steps": [VP, I32, I32],
+    "w2d_run_read": [VP, ctypes.POINTER(MemberState), I32],
+    "w2d_run_idle": [VP, VP],
+    "w2d_active_tiles": [VP, VP, ctypes.POINTER(I32)],
+    "w2d_tally_totals": [VP, VP],
     "w2d_copy_probe": [I64, I32, ctypes.POINTER(D)],
     "w2d_timing_read": [VP, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(I64)],
 }
@@ -117,7 +139,7 @@ class HipBackend:
 
     name = "hip"
 
-    def __init__(self, tables, num, precision: str = "f32", variant: str = "fast") -> None:
+    def __init__(self, tables, num, precision: str = "f32", variant: str = "fast", face_cache: bool = True) -> None:
         self.grid = tables.grid
         self.num = num
         self.rows = tables.rows
@@ -139,7 +161,8 @@ class HipBackend:
                     int(num.sheet), int(num.mirror), int(num.carry))  # fmt: skip
         handle = VP()
         check(api().w2d_create(ctypes.byref(t), ctypes.byref(p), F64 if precision == "f64" else F32,
-                               FAST if variant == "fast" else EXACT, ctypes.byref(handle)))  # fmt: skip
+                               (FAST if variant == "fast" else EXACT) | (0 if face_cache else 4),
+                               ctypes.byref(handle)))  # fmt: skip
         self.h = handle
         self.extra_wave = 0.0
         self.hooks = None
@@ -219,6 +242,10 @@ class HipBackend:
     def synchronize(self) -> None:
         check(api().w2d_synchronize(self.h))
 
+    def set_launch(self, faces_threads: int) -> None:
+        """Workgroup size of the face kernel (256 or 512); the results do not depend on it."""
+        check(api().w2d_set_launch(self.h, int(faces_threads)))
+
     # ---- the run's accumulators on the device (engine/maxima.py, engine/monitor.py)
     def tally_set(self, faces: np.ndarray, axis: np.ndarray, sign: np.ndarray) -> None:
         f, a, s = _c(faces, np.int32), _c(axis, np.int32), _c(sign, np.float32)
@@ -232,6 +259,12 @@ class HipBackend:
         out = D(0.0)
         check(api().w2d_tally_volume(self.h, ctypes.byref(out)))
         return float(out.value)
+
+    def tally_totals(self) -> np.ndarray:
+        out = np.zeros(self._tally_n, np.float64)
+        if self._tally_n:
+            check(api().w2d_tally_totals(self.h, out.ctypes.data))
+        return out
 
     def tally_flow(self) -> float:
         out = D(0.0)
@@ -269,6 +302,34 @@ class HipBackend:
         f, c, n = D(0.0), D(0.0), I64(0)
         check(api().w2d_timing_read(self.h, ctypes.byref(f), ctypes.byref(c), ctypes.byref(n)))
         return float(f.value), float(c.value), int(n.value)
+
+    # ---- device-resident run
+    def run_setup(self, members: list[Member], scales: np.ndarray, num: RunNumerics, idle: np.ndarray | None) -> None:
+        arr = (Member * len(members))(*members)
+        sc = _c(scales, np.float64)
+        idl = None if idle is None else _c(idle, np.int32)
+        check(api().w2d_run_setup(self.h, arr, len(members), sc.ctypes.data if sc.size else None, int(sc.size),
+                                  ctypes.byref(num), None if idl is None else idl.ctypes.data))  # fmt: skip
+        self._run_m = len(members)
+
+    def run_steps(self, nsteps: int, graph_steps: int = 0) -> None:
+        check(api().w2d_run_steps(self.h, int(nsteps), int(graph_steps)))
+
+    def run_read(self) -> list[MemberState]:
+        out = (MemberState * self._run_m)()
+        check(api().w2d_run_read(self.h, out, self._run_m))
+        n = I32(0)
+        check(api().w2d_active_tiles(self.h, None, ctypes.byref(n)))
+        tiles = np.zeros(max(n.value, 1), np.int32)
+        check(api().w2d_active_tiles(self.h, tiles.ctypes.data, ctypes.byref(n)))
+        self.active_tiles = tiles[: n.value].copy()
+        self.nact = int(n.value)
+        return list(out)
+
+    def run_idle(self) -> np.ndarray:
+        out = np.zeros(self.grid.ntiles, np.int32)
+        check(api().w2d_run_idle(self.h, out.ctypes.data))
+        return out
 
     def storage_set(self, index: np.ndarray) -> None:
         idx = _c(index, np.int64)

@@ -12,12 +12,12 @@ from wadi2d_amd.solver.stepper import Stepper
 
 
 class Model:
-    def __init__(self, tables, num: Numerics | None = None, backend: str = "auto") -> None:
+    def __init__(self, tables, num: Numerics | None = None, backend: str = "auto", **options) -> None:
         self.tables = tables
         self.grid = tables.grid
         self.num = num or Numerics()
         record = NumericsRecord.from_config(self.num, self.grid.dx)
-        self.backend = select_backend(backend, tables, record)
+        self.backend = select_backend(backend, tables, record, **options)
         self.stepper = Stepper(self.backend, self.grid, self.num)
         self.boundary_tiles = np.zeros(0, np.int64)
         self._hc = self.grid.from_tiles(tables.hc).astype(np.float64)
