@@ -55,6 +55,7 @@ def main() -> int:
     ap.add_argument("--graph", type=int, default=64, help="steps per captured graph (device loop)")
     ap.add_argument("--face-cache", choices=("on", "off"), default="on", help="per-face row cache of the face tables")
     ap.add_argument("--faces-threads", type=int, choices=(256, 512), default=512, help="face kernel workgroup size")
+    ap.add_argument("--cells-threads", type=int, choices=(256, 512), default=256, help="cell kernel workgroup size")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -84,7 +85,7 @@ def main() -> int:
     t0 = time.time()
     model = Model(tables, scen.numerics, backend, face_cache=args.face_cache == "on")
     model.set_depth(np.zeros(tables.grid.shape))
-    model.backend.set_launch(args.faces_threads)
+    model.backend.set_launch(args.faces_threads, args.cells_threads)
     drv = Driver(model, bset, rain)
     log(f"[bench] device upload: {time.time() - t0:.1f} s")
     b = model.backend
@@ -181,12 +182,25 @@ def main() -> int:
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": face_achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": measured_traffic(args, tiles),
         },
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     return 0
+
+
+def measured_traffic(args, tiles: int):
+    """HBM bytes per face-kernel launch from the committed PMC summary of this workload (FETCH_SIZE and WRITE_SIZE,
+    each scaled by what the counter reads for a copy of known size), or None for another workload or launch."""
+    path = os.path.join(ROOT, "profiles", "traffic_full_extent_4096.json")
+    if args.size != 4096 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rec = json.load(f)
+    if tiles != 16641 or rec["launch"]["faces_threads"] != args.faces_threads:
+        return None
+    return rec["per_launch"]["faces_kernel"]["bytes"]
 
 
 def cpu_baseline(model, tables, num, seconds: float, dt: float):

@@ -147,9 +147,10 @@ int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
 int w2d_run_idle(w2d_handle* h, int32_t* out);
 /* The active list as the device last left it (out may be null to ask for the count). */
 int w2d_active_tiles(w2d_handle* h, int32_t* out, int32_t* n);
-/* Workgroup size of the face kernel, 256 or 512 threads (default 512); the results do not depend on it. No
- * counterpart in the reference, whose launch shape is fixed (gpu/backend.py:151-188): a tuning knob. */
-int w2d_set_launch(w2d_handle* h, int32_t faces_threads);
+/* Workgroup sizes of the face and cell kernels, 256 or 512 threads each (defaults 512, 256); the fields do not
+ * depend on them. No counterpart in the reference, whose launch shape is fixed (gpu/backend.py:151-188): the
+ * launch-shape tuner's knob. */
+int w2d_set_launch(w2d_handle* h, int32_t faces_threads, int32_t cells_threads);
 /* synchronize (gpu/backend.py:202-203). */
 int w2d_synchronize(w2d_handle* h);
 const char* w2d_last_error(void);

@@ -117,3 +117,17 @@ def test_rain_crest_and_free_outflow_with_adaptive_stepping(backend, prec, tol):
         assert np.array_equal(np.sort(hip.backend.active_tiles), np.sort(ora.backend.active_tiles))
         _compare(hip, ora, tol, f"{backend} valley t={t_end}")
     assert hip.stepper.nsteps > 50
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("faces_threads, cells_threads", [(256, 256), (256, 512), (512, 512)])
+def test_every_launch_shape_is_bitwise_the_oracle(faces_threads, cells_threads):
+    """The workgroup sizes change which thread stages and computes which cell, never the arithmetic."""
+    tables, h0 = rough_dam_break()
+    num = Numerics(dt_every=1, dt_max=0.2)
+    hip, ora = _pair(tables, num, "hip-exact", 4, h0=h0)
+    hip.backend.set_launch(faces_threads, cells_threads)
+    hip.advance(5.0)
+    ora.advance(5.0)
+    assert hip.stepper.nsteps == ora.stepper.nsteps
+    _compare(hip, ora, 0.0, f"launch {faces_threads}/{cells_threads}")

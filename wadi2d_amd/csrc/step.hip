@@ -705,15 +705,15 @@ __device__ __forceinline__ R scaled(R q, R rl, R rr) { return q * (q > R(0) ? rl
 
 // Positivity ratio on the tile and a one-cell ring (cells.cu:4-18), then continuity, rain, the compensated
 // residual, the level and wet share of every own cell (cells.cu:28-63).
-template <class R, bool FAST>
-__global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
+template <class R, bool FAST, int NTH>
+__global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     __shared__ __align__(16) R s_h[LSIZE];
     __shared__ __align__(16) R s_qx[LSIZE];
     __shared__ __align__(16) R s_qy[LSIZE];
     __shared__ __align__(16) R s_r[LSIZE];
     __shared__ int nb[9];
     __shared__ int any_wet;
-    __shared__ double red_src[NT / 64];
+    __shared__ double red_src[NTH / 64];
     const int slot = xcd_slot(blockIdx.x, A.members ? A.glob->nact : A.nact);
     if (slot < 0) return;
     const int t = A.active[slot];
@@ -733,17 +733,17 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     if (threadIdx.x == 0) any_wet = 0;
     __syncthreads();
     {
-        const Stager<NT> sg = make_stager<NT>(t, nb);
+        const Stager<NTH> sg = make_stager<NTH>(t, nb);
         R* const dst[3] = {s_h, s_qx, s_qy};
         const R* const src[3] = {A.s.h, A.s.qx1, A.s.qy1};
-        stage<R, 3, NT>(dst, src, sg);
+        stage<R, 3, NTH>(dst, src, sg);
     }
     __syncthreads();
     const R dx = R(A.P.dx);
     const R area = dx * dx;
     const R dt = dt_run;
     constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32
-    for (int p = threadIdx.x; p < RW * RW; p += NT) {
+    for (int p = threadIdx.x; p < RW * RW; p += NTH) {
         const int jj = p / RW, ii = p - (p / RW) * RW;
         const int L = lds_of(jj - 1, ii - 1);
         const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + LS], R(0)) +
@@ -757,8 +757,8 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     int wet = 0;
     double src_sum = 0.0;
 #pragma unroll 1
-    for (int r = 0; r < ROWS_PER_THREAD; ++r) {
-        const int j = (threadIdx.x >> 5) + r * (NT / TILE);
+    for (int r = 0; r < TILE2 / NTH; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
         const R rc = s_r[L];
@@ -798,7 +798,11 @@ __global__ __launch_bounds__(NT) void cells_kernel(StepArgs<R> A) {
     __syncthreads();
     if (threadIdx.x == 0) {
         A.wet[t] = any_wet;
-        if (new_window) A.src_partial[t] = red_src[0] + red_src[1] + red_src[2] + red_src[3];
+        if (new_window) {
+            double sum = red_src[0];
+            for (int w = 1; w < NTH / 64; ++w) sum += red_src[w];
+            A.src_partial[t] = sum;
+        }
     }
 }
 
@@ -1309,6 +1313,7 @@ struct w2d_handle {
     int fast = 0;
     int face_cache = 1;
     int faces_threads = 512;
+    int cells_threads = 256;
     int ntiles = 0;
     int rows = 0;
     size_t ncell = 0;
@@ -1483,14 +1488,23 @@ int mark(w2d_handle* H) {
     return 0;
 }
 
-// The face kernel at the handle's workgroup size: 256 threads (4 cells each) or 512 (2 cells each; the same LDS
-// per workgroup, twice the waves per CU).
+// The face and cell kernels at the handle's workgroup sizes: 256 threads (4 cells each) or 512 (2 cells each; the
+// same LDS per workgroup, twice the waves per CU).
 template <class R, bool FAST>
 int launch_faces(w2d_handle* H, const StepArgs<R>& A, int blocks) {
     if (H->faces_threads == 512)
         faces_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
     else
         faces_kernel<R, FAST, 256><<<blocks, 256, 0, H->stream>>>(A);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+template <class R, bool FAST>
+int launch_cells(w2d_handle* H, const StepArgs<R>& A, int blocks) {
+    if (H->cells_threads == 512)
+        cells_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
+    else
+        cells_kernel<R, FAST, 256><<<blocks, 256, 0, H->stream>>>(A);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -1502,8 +1516,7 @@ int launch_step(w2d_handle* H, double dt, int want_wave) {
     if (H->timing && (s = mark(H))) return s;
     if ((s = launch_faces<R, FAST>(H, A, H->nact))) return s;
     if (H->timing && (s = mark(H))) return s;
-    cells_kernel<R, FAST><<<H->nact, NT, 0, H->stream>>>(A);
-    HIP_TRY(hipGetLastError());
+    if ((s = launch_cells<R, FAST>(H, A, H->nact))) return s;
     if (H->timing && (s = mark(H))) return s;
     if (H->nb_cells) {
         boundary_kernel<R><<<(H->nb_cells + 255) / 256, 256, 0, H->stream>>>(
@@ -2104,15 +2117,18 @@ int w2d_copy_probe(int64_t bytes, int32_t repeats, double* ms) {
     return 0;
 }
 
-int w2d_set_launch(w2d_handle* H, int32_t faces_threads) {
+int w2d_set_launch(w2d_handle* H, int32_t faces_threads, int32_t cells_threads) {
     if (!H) return fail(1, "set_launch: no handle");
     if (faces_threads != 256 && faces_threads != 512) return fail(1, "set_launch: faces_threads is 256 or 512");
-    if (faces_threads != H->faces_threads && H->graph) {  // a captured graph holds the old launches
+    if (cells_threads != 256 && cells_threads != 512) return fail(1, "set_launch: cells_threads is 256 or 512");
+    if ((faces_threads != H->faces_threads || cells_threads != H->cells_threads) && H->graph) {
+        // a captured graph holds the old launches
         (void)hipGraphExecDestroy(H->graph);
         H->graph = nullptr;
         H->graph_steps = 0;
     }
     H->faces_threads = faces_threads;
+    H->cells_threads = cells_threads;
     return 0;
 }
 
@@ -2160,7 +2176,7 @@ int enqueue_run_step(w2d_handle* H) {
     if (H->timing && (s = mark(H))) return s;
     if ((s = launch_faces<R, FAST>(H, A, nt))) return s;
     if (H->timing && (s = mark(H))) return s;
-    cells_kernel<R, FAST><<<nt, NT, 0, H->stream>>>(A);
+    if ((s = launch_cells<R, FAST>(H, A, nt))) return s;
     if (H->timing && (s = mark(H))) return s;
     post_kernel<R><<<H->run_m, 256, 0, H->stream>>>(H->run_mem, (const R*)H->wave, H->run_is_active,
                                                      H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,

@@ -76,7 +76,7 @@ _SIGS = {
     "w2d_storage_set": [VP, VP, I64],
     "w2d_storage": [VP, ctypes.POINTER(D)],
     "w2d_synchronize": [VP],
-    "w2d_set_launch": [VP, I32],
+    "w2d_set_launch": [VP, I32, I32],
     "w2d_source_weights": [VP, VP, VP, I64],
     "w2d_source_scale": [VP, D],
     "w2d_source_sum": [VP, ctypes.POINTER(D)],
@@ -242,9 +242,9 @@ class HipBackend:
     def synchronize(self) -> None:
         check(api().w2d_synchronize(self.h))
 
-    def set_launch(self, faces_threads: int) -> None:
-        """Workgroup size of the face kernel (256 or 512); the results do not depend on it."""
-        check(api().w2d_set_launch(self.h, int(faces_threads)))
+    def set_launch(self, faces_threads: int = 512, cells_threads: int = 256) -> None:
+        """Workgroup sizes of the face and cell kernels (256 or 512 each); the fields do not depend on them."""
+        check(api().w2d_set_launch(self.h, int(faces_threads), int(cells_threads)))
 
     # ---- the run's accumulators on the device (engine/maxima.py, engine/monitor.py)
     def tally_set(self, faces: np.ndarray, axis: np.ndarray, sign: np.ndarray) -> None:
