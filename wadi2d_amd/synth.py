@@ -2,7 +2,7 @@
 
 The reference defines none of these beds; they are this package's, fixed here and pinned by tests:
 
-* ``plain_cpu_case`` -- 256 x 256 cells at 10 m. Ground on a 1 m sub-grid (10 x 10 pixels per cell): a plane
+* ``plain_cpu_case`` -- 256 x 256 cells at 10 m (``c0_style_case``, which also makes the ensemble's members). Ground on a 1 m sub-grid (10 x 10 pixels per cell): a plane
   falling 0.5 % toward the east, plus 6-octave value noise (amplitude 2 m, base wavelength 640 m, persistence 0.5),
   plus a straight north-south wall 1.5 m high and 2 pixels thick at x = 1280 m with one 20 m gap centred on the
   domain. 8 table rows, Manning n 0.04, uniform rain 50 mm/h for 30 min then dry, 2 h, closed walls but a free
@@ -73,19 +73,24 @@ def _pixel_centres(grid: TileGrid, k: int):
     return x, y, p
 
 
-def plain_cpu_case(seed: int = 0) -> Scenario:
-    n, dx, k = 256, 10.0, 10
+def c0_style_case(n: int, dx: float, k: int, seed: int, storm, duration_s: float, name: str) -> Scenario:
+    """n x n cells: a plane falling 0.5 % east, 6-octave value noise (2 m, base 640 m), a wall 1.5 m high and 2
+    pixels thick across the middle (x = n dx / 2) with one 20 m gap centred on the domain; 8 rows, n 0.04, a free
+    east column."""
     grid = TileGrid.from_mask(np.ones((n, n), bool), 0.0, n * dx, dx)
     x, y, p = _pixel_centres(grid, k)
     z = np.zeros((y.size, x.size))
     noise(z, grid.x0, grid.y0, p, seed, 6, 640.0, 0.5, 2.0)
     z += 0.005 * (n * dx - x)[None, :]
-    wall = (x >= 1280.0) & (x < 1280.0 + 2 * p)
+    wall = (x >= 0.5 * n * dx) & (x < 0.5 * n * dx + 2 * p)
     gap = np.abs(y - 0.5 * n * dx) < 10.0
     z[np.ix_(~gap, wall)] += 1.5
     east = [Boundary("free", np.arange(1, n + 1), np.full(n, n))]
-    return Scenario("plain 256x256 dx10", grid, ArrayBed(z, p), Tables(rows=8), 0.04, east,
-                    uniform_storm(25.0, 1800.0), None, 7200.0, Numerics())  # fmt: skip
+    return Scenario(name, grid, ArrayBed(z, p), Tables(rows=8), 0.04, east, storm, None, duration_s, Numerics())
+
+
+def plain_cpu_case(seed: int = 0) -> Scenario:
+    return c0_style_case(256, 10.0, 10, seed, uniform_storm(25.0, 1800.0), 7200.0, "plain 256x256 dx10")
 
 
 def _walks(rng, nsteps: int, extent: float, pixel: float, count: int) -> list[np.ndarray]:
