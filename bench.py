@@ -60,6 +60,9 @@ def main() -> int:
                     help="step: the 4096x4096 plain step (default); ensemble: the 64-member storm/ARF ensemble")
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
     ap.add_argument("--serial-members", type=int, default=-1, help="members the serial baseline runs (-1: all)")
+    ap.add_argument("--reference", choices=("on", "off"), default="on", help="the FP64 batched run (ensemble)")
+    ap.add_argument("--maxima-every", type=int, default=4,
+                    help="ensemble: sample the running maxima every N steps (the run loop's default, 4)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus})", file=sys.stderr)
@@ -207,18 +210,22 @@ def ensemble(args) -> int:
     ens = Ensemble.build(specs)
     log(f"[bench] {len(specs)} members, {ens.cells} cells, {ens.tables.grid.ntiles - 1} tiles: built in "
         f"{time.time() - t0:.1f} s")
-    ref = run_batched(ens, "hip-f64", tune=False, launch=(512, 256), log=log)
-    log(f"[bench] FP64 batched: {ref.wall_s:.1f} s, steps {min(ref.steps)}..{max(ref.steps)}")
-    got = run_batched(ens, "hip", log=log)
+    ref = None
+    if args.reference == "on":
+        ref = run_batched(ens, "hip-f64", graph_steps=args.graph, tune=False, launch=(512, 256),
+                          maxima_every=args.maxima_every, log=log)  # fmt: skip
+        log(f"[bench] FP64 batched: {ref.wall_s:.1f} s, steps {min(ref.steps)}..{max(ref.steps)}")
+    got = run_batched(ens, "hip", graph_steps=args.graph, maxima_every=args.maxima_every, log=log)
     log(f"[bench] FP32 batched: {got.wall_s:.1f} s, launch {got.launch}, tuning {got.tuning}")
     n_serial = len(specs) if args.serial_members < 0 else min(args.serial_members, len(specs))
     members = list(range(n_serial))
-    plain = run_serial(ens, "hip", members, log=log)
+    plain = run_serial(ens, "hip", members, args.maxima_every, log=log)
     hours = [sc.duration_s / 3600.0 for sc in ens.scenarios]
     batched_rate = sum(hours) / (got.wall_s / 3600.0)
-    plain_rate = sum(hours[m] for m in members) / (plain.wall_s / 3600.0)
-    err_b = max_depth_error(got.max_depth, ref.max_depth, ens)
-    err_p = max_depth_error(plain.max_depth, [ref.max_depth[m] for m in members], ens, members)
+    plain_rate = sum(hours[m] for m in members) / (plain.wall_s / 3600.0) if members else None
+    err_b = max_depth_error(got.max_depth, ref.max_depth, ens) if ref else [None]
+    err_p = max_depth_error(plain.max_depth, [ref.max_depth[m] for m in members], ens, members) if ref and members \
+        else [None]
     out = {
         "metric": "ensemble member-hours per wall-hour",
         "value": batched_rate,
@@ -236,15 +243,16 @@ def ensemble(args) -> int:
             "workload": f"{len(specs)} members, sides from 128..1024 at 20 m ({ens.cells} cells), own seeded ground "
                         "each, NRCS Type II storms of 20-120 mm over 1-6 h times an ARF of 0.6-1.0, 3 h each",
             "launch": list(got.launch) if got.launch else None,
+            "maxima_every": args.maxima_every,
             "plain_member_hours_per_hour": plain_rate,
             "plain_members_timed": n_serial,
-            "speedup_vs_plain": batched_rate / plain_rate,
+            "speedup_vs_plain": batched_rate / plain_rate if plain_rate else None,
             "max_depth_error_batched_m": max(err_b),
             "max_depth_error_plain_m": max(err_p),
             "mass_error_max": max(abs(e) for e in got.mass_error),
             "batched_wall_s": got.wall_s,
             "plain_wall_s": plain.wall_s,
-            "fp64_wall_s": ref.wall_s,
+            "fp64_wall_s": ref.wall_s if ref else None,
         },
         "roofline": None,
         "cpu_baseline": None,

@@ -126,6 +126,8 @@ typedef struct w2d_member {
     int64_t nsteps0;
     int32_t refresh_due;
 } w2d_member;
+/* maxima: the running maxima are sampled after every step whose count is a multiple of it (0: not sampled), as
+ * the run loop's maxima_every (engine/maxima.py:103-146, config/schema.py:1001). */
 typedef struct w2d_run_numerics {
     double cfl, dt_max, dt_min, wave_floor;
     int32_t dt_every, active_every, maxima;

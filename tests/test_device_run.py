@@ -81,7 +81,7 @@ def test_plain_case_through_the_device_loop_meets_the_fp64_oracle():
     m = Model(tables, scen.numerics, "hip-f64")
     m.set_depth(np.zeros(tables.grid.shape))
     Driver(m, bset, rain, maxima_every=1)
-    run = DeviceRun(m.backend, [_device_plan(m, bset, rain, scen.duration_s)], m.num, maxima=True,
+    run = DeviceRun(m.backend, [_device_plan(m, bset, rain, scen.duration_s)], m.num, maxima=1,
                     idle=m.stepper.idle)  # fmt: skip
     s = run.run(chunk=512, graph_steps=64)[0]
     bal = run.balance(0, m.backend.tally_volume())

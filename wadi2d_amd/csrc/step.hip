@@ -603,12 +603,12 @@ struct Member {
     int tile0, tile1;
     int b0, b1, f0, f1;  // the member's boundary cells and tally faces in the handle's lists
     int stepping, refresh, refresh_due, new_window, dry, rebuild, check, done, failed;
+    int nlist;  // the member's active tiles, kept at seg[tile0 .. tile0 + nlist)
 };
 
 struct RunGlobals {
     int nact;
-    int any_rebuild;
-    int n_woken, n_leaving;
+    int changed;  // a member's list changed since the active list was last gathered
 };
 
 struct RunNum {
@@ -965,10 +965,12 @@ __global__ void scale_source_kernel(size_t n, const double* __restrict__ w, doub
 
 // ------------------------------------------------------------------------------------------------------------
 // device-resident run: the time loop of wadi2d/solver/stepper.py:101-129 and the window loop of
-// wadi2d/run.py:1712-1739, per member, with no host in the loop. A step is four launches: `pre_kernel` (one
-// workgroup: the active-list check the last step asked for, the clocks and source windows, the list change they
-// ask for, woken and leaving tiles), the face and cell kernels, and `post_kernel` (one workgroup per member: its
-// boundary cells, tally, wave-speed reduction, rain volume, clock and next step length).
+// wadi2d/run.py:1712-1739, per member, with no host in the loop. A step is five launches: `pre_kernel` (one
+// workgroup per member: the active-list check the last step asked for, the clock and source window, the list
+// change they ask for, woken and leaving tiles), `gather_kernel` (the members' lists end to end, when one changed),
+// the face and cell kernels, and `post_kernel` (one workgroup per member: its boundary cells, tally, wave-speed
+// reduction, rain volume, clock and next step length). A member's tiles are one contiguous range and no tile has a
+// neighbour in another member, so members never touch each other's tiles.
 
 constexpr int SLEEP_CHECKS = 2;  // solver/stepper.py:14
 constexpr int PRE_THREADS = 1024;
@@ -1032,7 +1034,6 @@ __device__ __forceinline__ int chunk_slot(bool flag, int* wave_cnt, int& base) {
 }
 
 struct RunLists {
-    const int* tile_member;
     const int* nbr;
     const int* wet;
     const int* has_source;
@@ -1040,80 +1041,67 @@ struct RunLists {
     int* is_active;
     int* idle;
     int* flags;
+    int* seg;      // per member, its active tiles from seg[tile0]
+    int* woken;    // per member, from woken[tile0]
+    int* leaving;  // per member, from leaving[tile0]
     int* active;
-    int* woken;
-    int* leaving;
-    int ntiles;
 };
 
-// The active list (stepper.py:56-92) per member, by its mode: 1 keeps what is listed and adds wet and kept tiles
-// with their neighbours (retain); 2 does that and lets a tile go after SLEEP_CHECKS checks that did not want it
-// (check); 3 drops the member's tiles (done); 0 leaves them. One workgroup, so the list stays in ascending order.
-// Woken and leaving tiles go to their lists; returns nothing, writes nact, n_woken, n_leaving.
-template <bool CHECK>
-__device__ void rebuild(const Member* __restrict__ mem, RunGlobals* __restrict__ g, const RunLists& L, int* wave_cnt) {
-    auto mode_of = [&](int t) {
-        const Member& b = mem[L.tile_member[t]];
-        return CHECK ? (b.check ? 2 : 0) : b.rebuild;
-    };
-    for (int t = threadIdx.x; t < L.ntiles; t += PRE_THREADS) {
+// One member's active tiles (stepper.py:56-92) by `mode`: 1 keeps what is listed and adds wet and kept tiles with
+// their neighbours (retain); 2 does that and lets a tile go after SLEEP_CHECKS checks that did not want it (check);
+// 3 drops the member's tiles (done). The list stays in ascending order; woken and leaving tiles go to their lists.
+// One workgroup, the member's own tiles.
+__device__ void rebuild(Member& b, int mode, const RunLists& L, int* wave_cnt, int& n_woken, int& n_leaving) {
+    const int t0 = b.tile0, t1 = b.tile1;
+    const bool look = mode == 1 || mode == 2;
+    for (int t = t0 + threadIdx.x; t < t1; t += PRE_THREADS) {
         int wanted = 0;
-        if (t > 0) {
-            const int mode = mode_of(t);
-            if (mode == 1 || mode == 2) {
-                const Member& b = mem[L.tile_member[t]];
-                const int keep = (L.has_source[t] && b.scale != 0.0) || L.bnd_tile[t];
-                wanted = L.wet[t] || keep;
-            }
+        if (look) {
+            const int keep = (L.has_source[t] && b.scale != 0.0) || L.bnd_tile[t];
+            wanted = L.wet[t] || keep;
         }
         L.flags[t] = wanted;
     }
     __syncthreads();
-    for (int t = threadIdx.x; t < L.ntiles; t += PRE_THREADS) {
+    for (int t = t0 + threadIdx.x; t < t1; t += PRE_THREADS) {
         const int cur = L.is_active[t];
-        int a = 0;
-        if (t > 0) {
-            const int mode = mode_of(t);
-            a = cur;
-            if (mode == 3) {
-                a = 0;
-            } else if (mode == 1 || mode == 2) {
-                a = L.flags[t] & 1;
-                for (int q = 0; q < 8; ++q) a |= L.flags[L.nbr[(size_t)t * 8 + q]] & 1;
-                if (mode == 1) {
-                    a |= cur;
-                } else {
-                    L.idle[t] = a ? 0 : L.idle[t] + 1;
-                    a |= cur && L.idle[t] < SLEEP_CHECKS;
-                }
+        int a = cur;
+        if (mode == 3) {
+            a = 0;
+        } else if (look) {
+            // neighbours are this member's tiles or the void tile, whose flag is never set
+            a = L.flags[t] & 1;
+            for (int q = 0; q < 8; ++q) a |= L.flags[L.nbr[(size_t)t * 8 + q]] & 1;
+            if (mode == 1) {
+                a |= cur;
+            } else {
+                L.idle[t] = a ? 0 : L.idle[t] + 1;
+                a |= cur && L.idle[t] < SLEEP_CHECKS;
             }
         }
         L.flags[t] |= a << 1;
     }
     __syncthreads();
     int na = 0, nw = 0, nl = 0;
-    for (int base = 0; base < L.ntiles; base += PRE_THREADS) {
+    for (int base = t0; base < t1; base += PRE_THREADS) {
         const int t = base + threadIdx.x;
-        const bool in = t < L.ntiles;
+        const bool in = t < t1;
         const int a = in ? (L.flags[t] >> 1) & 1 : 0;
         const int cur = in ? L.is_active[t] : 0;
-        const int mode = (in && t > 0) ? mode_of(t) : 0;
-        const bool wk = a && !cur && (mode == 1 || mode == 2);
+        const bool wk = a && !cur && look;
         const bool lv = cur && !a;
         const int sa = chunk_slot(a, wave_cnt, na);
         const int sw = chunk_slot(wk, wave_cnt, nw);
         const int sl = chunk_slot(lv, wave_cnt, nl);
-        if (a) L.active[sa] = t;
-        if (wk) L.woken[sw] = t;
-        if (lv) L.leaving[sl] = t;
+        if (a) L.seg[t0 + sa] = t;
+        if (wk) L.woken[t0 + sw] = t;
+        if (lv) L.leaving[t0 + sl] = t;
         if (in) L.is_active[t] = a;
     }
     __syncthreads();
-    if (threadIdx.x == 0) {
-        g->nact = na;
-        g->n_woken = nw;
-        g->n_leaving = nl;
-    }
+    if (threadIdx.x == 0) b.nlist = na;
+    n_woken = nw;
+    n_leaving = nl;
     __syncthreads();
 }
 
@@ -1121,29 +1109,27 @@ __device__ void rebuild(const Member* __restrict__ mem, RunGlobals* __restrict__
 // kernel's own arithmetic; a tile that leaves keeps one mean depth in both buffers, and a finished member's tile
 // also its face buffers.
 template <class R, bool FAST>
-__device__ void transitions(const RunGlobals* __restrict__ g, const Member* __restrict__ mem, const RunLists& L,
-                            R* __restrict__ h, R* __restrict__ h_other, R* __restrict__ om,
-                            R* const* __restrict__ faces_cur, R* const* __restrict__ faces_other,
-                            const float* __restrict__ hc, const float2* __restrict__ ctab, size_t ncell, int last) {
-    const int nw = g->n_woken, nl = g->n_leaving;
-    for (int b = 0; b < nw; ++b) {
-        const size_t c = (size_t)L.woken[b] * TILE2 + threadIdx.x;
+__device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, R* __restrict__ h,
+                            R* __restrict__ h_other, R* __restrict__ om, R* const* __restrict__ faces_cur,
+                            R* const* __restrict__ faces_other, const float* __restrict__ hc,
+                            const float2* __restrict__ ctab, size_t ncell, int last) {
+    for (int q = 0; q < nw; ++q) {
+        const size_t c = (size_t)L.woken[b.tile0 + q] * TILE2 + threadIdx.x;
         om[c] = cell_plan<R, FAST>(h[c], R(hc[c]), ctab + c, ncell, last);
     }
-    for (int b = 0; b < nl; ++b) {
-        const int t = L.leaving[b];
-        const size_t c = (size_t)t * TILE2 + threadIdx.x;
+    for (int q = 0; q < nl; ++q) {
+        const size_t c = (size_t)L.leaving[b.tile0 + q] * TILE2 + threadIdx.x;
         h_other[c] = h[c];
-        if (mem[L.tile_member[t]].done)
+        if (b.done)
             for (int f = 0; f < 4; ++f) faces_other[f][c] = faces_cur[f][c];
     }
     __syncthreads();
 }
 
-// Before a step, one workgroup: the check the last step asked for (stepper.py:113-114), the clocks and windows,
-// the list change those ask for, and the tiles that woke or left.
+// Before a step, one workgroup per member: the check the last step asked for (stepper.py:113-114), the clock and
+// window, the list change those ask for, and the tiles that woke or left.
 template <class R, bool FAST>
-__global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ mem, int M, RunGlobals* __restrict__ g,
+__global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ mem, RunGlobals* __restrict__ g,
                                                           const double* __restrict__ scales, const RunNum num,
                                                           RunLists L, R* __restrict__ h, R* __restrict__ h_other,
                                                           R* __restrict__ om, R* const* __restrict__ faces_cur,
@@ -1153,29 +1139,47 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
                                                           int flush_only) {
     static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
     __shared__ int wave_cnt[PRE_THREADS / 64];
-    __shared__ int any_check, any;
-    if (threadIdx.x == 0) {
-        any_check = 0;
-        any = 0;
+    __shared__ int any;
+    Member& b = mem[blockIdx.x];
+    int nw, nl;
+    bool changed = false;
+    if (b.check) {
+        rebuild(b, 2, L, wave_cnt, nw, nl);
+        transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+        if (threadIdx.x == 0) b.check = 0;
+        changed = true;
     }
-    __syncthreads();
-    for (int m = threadIdx.x; m < M; m += PRE_THREADS)
-        if (mem[m].check) any_check = 1;
-    __syncthreads();
-    if (any_check) {
-        rebuild<true>(mem, g, L, wave_cnt);
-        transitions<R, FAST>(g, mem, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
-        for (int m = threadIdx.x; m < M; m += PRE_THREADS) mem[m].check = 0;
+    if (!flush_only) {
+        if (threadIdx.x == 0) {
+            int mine = 0;
+            ctl_begin(b, scales, num, mine);
+            any = mine;
+        }
         __syncthreads();
+        if (any) {
+            rebuild(b, b.rebuild, L, wave_cnt, nw, nl);
+            transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+            changed = true;
+        }
     }
-    if (flush_only) return;
-    int mine = 0;
-    for (int m = threadIdx.x; m < M; m += PRE_THREADS) ctl_begin(mem[m], scales, num, mine);
-    if (mine) any = 1;
+    if (changed && threadIdx.x == 0) g->changed = 1;
+}
+
+// The members' lists end to end, in member (so ascending tile) order, when one of them changed.
+__global__ __launch_bounds__(PRE_THREADS) void gather_kernel(const Member* __restrict__ mem, int M,
+                                                             RunGlobals* __restrict__ g, const int* __restrict__ seg,
+                                                             int* __restrict__ active) {
+    if (!g->changed) return;
+    int off = 0;
+    for (int m = 0; m < M; ++m) {
+        const Member& b = mem[m];
+        for (int k = threadIdx.x; k < b.nlist; k += PRE_THREADS) active[off + k] = seg[b.tile0 + k];
+        off += b.nlist;
+    }
     __syncthreads();
-    if (any) {
-        rebuild<false>(mem, g, L, wave_cnt);
-        transitions<R, FAST>(g, mem, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+    if (threadIdx.x == 0) {
+        g->nact = off;
+        g->changed = 0;
     }
 }
 
@@ -1267,11 +1271,12 @@ __global__ __launch_bounds__(NT) void maxima_run_kernel(const RunGlobals* __rest
                                                         const R* __restrict__ h, const R* __restrict__ u,
                                                         const R* __restrict__ v, R* __restrict__ dmax,
                                                         R* __restrict__ hmax, R* __restrict__ smax,
-                                                        R* __restrict__ zmax, R* __restrict__ tarr) {
+                                                        R* __restrict__ zmax, R* __restrict__ tarr, int every) {
     if ((int)blockIdx.x >= g->nact) return;
     const int t = active[blockIdx.x];
     const Member& b = mem[tile_member[t]];
-    if (!b.stepping) return;
+    // sampled after the steps whose count is a multiple of `every`, as the host loop's on_step does
+    if (!b.stepping || b.nsteps % every != 0) return;
     const R time = (R)b.t;
     const int te = nbr[(size_t)t * 8 + 4], ts = nbr[(size_t)t * 8 + 6];
     for (int p = threadIdx.x; p < TILE2; p += NT) {
@@ -1363,7 +1368,7 @@ struct w2d_handle {
     RunGlobals* run_glob = nullptr;
     RunNum run_num{};
     int *run_tile_member = nullptr, *run_has_source = nullptr, *run_bnd_tile = nullptr, *run_is_active = nullptr,
-        *run_idle = nullptr, *run_flags = nullptr, *run_woken = nullptr, *run_leaving = nullptr;
+        *run_idle = nullptr, *run_flags = nullptr, *run_woken = nullptr, *run_leaving = nullptr, *run_seg = nullptr;
     double *run_scales = nullptr, *run_src_partial = nullptr;
     void** run_faces_ptrs = nullptr;  // [parity][cur 4, other 4]
     int run_maxima = 0;
@@ -2145,18 +2150,19 @@ int w2d_synchronize(w2d_handle* H) {
 
 namespace {
 
-template <class R>
 RunLists run_lists(w2d_handle* H) {
-    return RunLists{H->run_tile_member, H->nbr, H->wet, H->run_has_source, H->run_bnd_tile, H->run_is_active,
-                    H->run_idle, H->run_flags, H->active, H->run_woken, H->run_leaving, H->ntiles};
+    return RunLists{H->nbr, H->wet, H->run_has_source, H->run_bnd_tile, H->run_is_active, H->run_idle,
+                    H->run_flags, H->run_seg, H->run_woken, H->run_leaving, H->active};
 }
 
 template <class R, bool FAST>
 int enqueue_pre(w2d_handle* H, int flush_only) {
     R* const* fc = (R* const*)(H->run_faces_ptrs + 8 * H->ucur);  // current u, v, mx, my, then the others
-    pre_kernel<R, FAST><<<1, PRE_THREADS, 0, H->stream>>>(
-        H->run_mem, H->run_m, H->run_glob, H->run_scales, H->run_num, run_lists<R>(H), (R*)H->h[H->hcur],
-        (R*)H->h[1 - H->hcur], (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, flush_only);
+    pre_kernel<R, FAST><<<H->run_m, PRE_THREADS, 0, H->stream>>>(
+        H->run_mem, H->run_glob, H->run_scales, H->run_num, run_lists(H), (R*)H->h[H->hcur], (R*)H->h[1 - H->hcur],
+        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, flush_only);
+    HIP_TRY(hipGetLastError());
+    gather_kernel<<<1, PRE_THREADS, 0, H->stream>>>(H->run_mem, H->run_m, H->run_glob, H->run_seg, H->active);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2186,7 +2192,7 @@ int enqueue_run_step(w2d_handle* H) {
         maxima_run_kernel<R><<<nt, NT, 0, H->stream>>>(
             H->run_glob, H->run_mem, H->run_tile_member, H->active, H->nbr, (R)H->m_arrive, (R)H->m_deep, (R*)H->d,
             h_new, (R*)H->uv[0][1 - H->ucur], (R*)H->uv[1][1 - H->ucur], (R*)H->maxima[0], (R*)H->maxima[1],
-            (R*)H->maxima[2], (R*)H->maxima[3], (R*)H->maxima[4]);
+            (R*)H->maxima[2], (R*)H->maxima[3], (R*)H->maxima[4], H->run_maxima);
     HIP_TRY(hipGetLastError());
     H->hcur = 1 - H->hcur;
     H->ucur = 1 - H->ucur;
@@ -2222,10 +2228,8 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         if (d.tile0 < 1 || d.tile1 > nt || d.tile0 >= d.tile1 || d.window <= 0.0 || d.nwindows < 0 ||
             d.scale_offset < 0 || d.scale_offset + d.nwindows > nscales)
             return fail(1, "run_setup: a member's tiles or windows are out of range");
-        for (int t = d.tile0; t < d.tile1; ++t) {
-            if (tm[t] || (m == 0 && t == 0)) return fail(1, "run_setup: members overlap");
-            tm[t] = m;
-        }
+        if (m > 0 && d.tile0 < members[m - 1].tile1) return fail(1, "run_setup: members overlap or are out of order");
+        for (int t = d.tile0; t < d.tile1; ++t) tm[t] = m;
         Member& b = mem[m];
         std::memset(&b, 0, sizeof(b));
         b.t = d.t0;
@@ -2270,7 +2274,22 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     };
     if ((s = ranges(H->b_cell, H->nb_cells, true, "boundary cells"))) return s;
     if ((s = ranges(H->t_face, H->n_tally, false, "tally faces"))) return s;
-    for (int t : H->active_host) is_active[t] = 1;
+    // each member's neighbours are its own tiles or the void tile: members never touch each other's tiles
+    for (int m = 0; m < M; ++m)
+        for (int t = members[m].tile0; t < members[m].tile1; ++t)
+            for (int q = 0; q < 8; ++q) {
+                const int u = H->nbr_host[(size_t)t * 8 + q];
+                if (u != 0 && (u < members[m].tile0 || u >= members[m].tile1))
+                    return fail(1, "run_setup: a member's tile has a neighbour outside the member");
+            }
+    std::vector<int32_t> seg(nt, 0);
+    for (int t : H->active_host) {
+        int m = 0;
+        while (m < M && !(t >= members[m].tile0 && t < members[m].tile1)) ++m;
+        if (m == M) return fail(1, "run_setup: an active tile belongs to no member");
+        is_active[t] = 1;
+        seg[members[m].tile0 + mem[m].nlist++] = t;
+    }
     std::vector<int32_t> idle_v(nt, 0);
     if (idle) idle_v.assign(idle, idle + nt);
     H->run_m = 0;
@@ -2285,7 +2304,7 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     if ((s = up(&H->run_tile_member, tm)) || (s = up(&H->run_has_source, has_source)) ||
         (s = up(&H->run_bnd_tile, bnd_tile)) || (s = up(&H->run_is_active, is_active)) ||
         (s = up(&H->run_idle, idle_v)) || (s = up(&H->run_flags, zeros)) || (s = up(&H->run_woken, zeros)) ||
-        (s = up(&H->run_leaving, zeros)))
+        (s = up(&H->run_leaving, zeros)) || (s = up(&H->run_seg, seg)))
         return s;
     if (H->run_mem) {
         H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_mem), H->owned.end());
@@ -2321,7 +2340,8 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     }
     H->run_num = RunNum{num->cfl, num->dt_max, num->dt_min, num->wave_floor, H->P.dx, H->P.dx * H->P.dx,
                         num->dt_every, num->active_every};
-    H->run_maxima = num->maxima != 0;
+    if (num->maxima < 0) return fail(1, "run_setup: maxima is an interval in steps (0: none)");
+    H->run_maxima = num->maxima;
     if (H->run_maxima && !H->maxima[0]) return fail(1, "run_setup: maxima asked for but maxima_set not called");
     if (H->graph) {
         (void)hipGraphExecDestroy(H->graph);

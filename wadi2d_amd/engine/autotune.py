@@ -23,14 +23,15 @@ def tune_launch(run, graph_steps: int = 64, chunk: int | None = None, rounds: in
     """Time every candidate (faces_threads, cells_threads) on the live run; set and return the fastest, with the
     measurements (seconds per active tile-step)."""
     b = run.b
-    steps = chunk or 2 * graph_steps
+    warm = graph_steps if graph_steps > 0 else 16
+    steps = chunk or 2 * warm
     log: list[dict] = []
     for _ in range(rounds):
         for c in candidates:
             if all(s.done for s in run.state):
                 break
             b.set_launch(*c)
-            run.steps(graph_steps, graph_steps)  # captures this shape's graph
+            run.steps(warm, graph_steps)  # captures this shape's graph
             before = b.nact
             t0 = time.perf_counter()
             run.steps(steps, graph_steps)

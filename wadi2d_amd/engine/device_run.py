@@ -47,8 +47,9 @@ def window_scales(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarr
 
 
 class DeviceRun:
-    def __init__(self, backend, plans: list[MemberPlan], num, source_every_s: float = 30.0, maxima: bool = False,
+    def __init__(self, backend, plans: list[MemberPlan], num, source_every_s: float = 30.0, maxima: int = 0,
                  idle: np.ndarray | None = None) -> None:  # fmt: skip
+        """``maxima``: sample the running maxima after every step whose count is a multiple of it (0: never)."""
         self.b = backend
         self.plans = plans
         self.area = backend.grid.dx ** 2

@@ -182,8 +182,9 @@ class BatchedResult:
 
 
 def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chunk: int = 512, tune: bool = True,
-                launch: tuple[int, int] | None = None, log=None) -> BatchedResult:  # fmt: skip
-    """Every member in one device-resident run to its end; per-member max depth (sampled every step), mass error."""
+                launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None) -> BatchedResult:  # fmt: skip
+    """Every member in one device-resident run to its end; per-member max depth (sampled every ``maxima_every``
+    steps), mass error."""
     from wadi2d_amd.engine.autotune import tune_launch
 
     num = ens.scenarios[0].numerics
@@ -207,7 +208,7 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
         rain = MemberRain(ens.specs[m].storm, ens.interior[m])
         plans.append(MemberPlan(sl.tile0, sl.tile1, rain, None, 0.0, ens.scenarios[m].duration_s, st.dt, 0, True,
                                 ens.interior[m]))  # fmt: skip
-    run = DeviceRun(b, plans, num, maxima=True, idle=st.idle)
+    run = DeviceRun(b, plans, num, maxima=maxima_every, idle=st.idle)
     b.synchronize()
     t0 = time.perf_counter()
     tuning = None
@@ -244,7 +245,8 @@ class SerialResult:
     mass_error: list[float]
 
 
-def run_serial(ens: Ensemble, backend: str = "hip", members: list[int] | None = None, log=None) -> SerialResult:
+def run_serial(ens: Ensemble, backend: str = "hip", members: list[int] | None = None, maxima_every: int = 1,
+               log=None) -> SerialResult:  # fmt: skip
     """The plain way: each member its own model, driven from the host one step at a time, one member after another
     (a storm study's serial loop, arf.py:347-371)."""
     which = range(len(ens.specs)) if members is None else members
@@ -257,7 +259,7 @@ def run_serial(ens: Ensemble, backend: str = "hip", members: list[int] | None = 
         t0 = time.perf_counter()
         model = Model(tables, sc.numerics, backend)
         model.set_depth(np.zeros(tables.grid.shape))
-        drv = Driver(model, bset, rain, maxima_every=1)
+        drv = Driver(model, bset, rain, maxima_every=maxima_every)
         drv.run(sc.duration_s)
         model.backend.synchronize()
         wall += time.perf_counter() - t0
