@@ -56,8 +56,9 @@ def main() -> int:
     ap.add_argument("--face-cache", choices=("on", "off"), default="on", help="per-face row cache of the face tables")
     ap.add_argument("--faces-threads", type=int, choices=(256, 512), default=512, help="face kernel workgroup size")
     ap.add_argument("--cells-threads", type=int, choices=(256, 512), default=256, help="cell kernel workgroup size")
-    ap.add_argument("--workload", choices=("step", "ensemble"), default="step",
-                    help="step: the 4096x4096 plain step (default); ensemble: the 64-member storm/ARF ensemble")
+    ap.add_argument("--workload", choices=("step", "ensemble", "precision"), default="step",
+                    help="step: the 4096x4096 plain step (default); ensemble: the 64-member storm/ARF ensemble; "
+                         "precision: FP32 against FP64 on the steep 2048x2048 reservoir break")  # fmt: skip
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
     ap.add_argument("--serial-members", type=int, default=-1, help="members the serial baseline runs (-1: all)")
     ap.add_argument("--reference", choices=("on", "off"), default="on", help="the FP64 batched run (ensemble)")
@@ -69,6 +70,8 @@ def main() -> int:
         return 2
     if args.workload == "ensemble":
         return ensemble(args)
+    if args.workload == "precision":
+        return precision(args)
 
     from wadi2d_amd import synth
     from wadi2d_amd.engine.driver import Driver
@@ -253,6 +256,63 @@ def ensemble(args) -> int:
             "batched_wall_s": got.wall_s,
             "plain_wall_s": plain.wall_s,
             "fp64_wall_s": ref.wall_s if ref else None,
+        },
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def precision(args) -> int:
+    """The steep 2048x2048 reservoir break, 1 h, in fast FP32 (compensated residual) and in FP64: max-depth, peak
+    outfall discharge and mass errors of the FP32 run against the FP64 one."""
+    from wadi2d_amd import synth
+    from wadi2d_amd.engine.precision import run_study
+
+    size = args.size if args.size != 4096 else 2048
+    t0 = time.time()
+    scen = synth.precision_case(size)
+    h0 = synth.reservoir_depth(scen)
+    scen.bed = None
+    log(f"[bench] {scen.name}: built in {time.time() - t0:.1f} s; reservoir {float(h0.sum()) * 100.0:.3e} m3")
+    ref = run_study(scen, h0, "hip-f64", log=log)
+    got = run_study(scen, h0, "hip", log=log)
+    mask = scen.tables.grid.mask
+    diff = np.abs(got.max_depth - ref.max_depth)[mask]
+    err = float(diff.max())
+    q_err = abs(got.peak_q - ref.peak_q) / max(abs(ref.peak_q), 1e-30)
+    cells = int(mask.sum())
+    wet = int((ref.max_depth[mask] > 0.01).sum())
+    out = {
+        "metric": "max-depth error FP32 against FP64",
+        "value": err,
+        "unit": "m",
+        "n_gpus": 1,
+        "steps": got.steps,
+        "warmup": 0,
+        "ms_per_step": got.wall_s / got.steps * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded steep ground, crests and reservoir built in-process)",
+        "config": {
+            "workload": f"{size}x{size} cells at 10 m, ridged ground (median slope 20 %), 40 crest breaklines, a 1 km2 "
+                        "reservoir 5 m above a crest that breaks at t=0, four free edges, 1 h",
+            "max_depth_error_p99_m": float(np.percentile(diff, 99)),
+            "max_depth_error_p999_m": float(np.percentile(diff, 99.9)),
+            "cells_over_2cm": int((diff > 0.02).sum()),
+            "cells_wet": wet,
+            "peak_q_fp64": ref.peak_q,
+            "peak_q_fp32": got.peak_q,
+            "peak_q_rel_error": q_err,
+            "mass_error_fp32": got.mass_error,
+            "mass_error_fp64": ref.mass_error,
+            "steps_fp64": ref.steps,
+            "cells": cells,
+            "wall_s_fp32": got.wall_s,
+            "wall_s_fp64": ref.wall_s,
         },
         "roofline": None,
         "cpu_baseline": None,

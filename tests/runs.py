@@ -25,13 +25,14 @@ class RunResult:
     h: np.ndarray
 
 
-def run_oracle(scen, prec: int = 8, threads: int = 8, t_end: float | None = None) -> RunResult:
-    """The run loop of run._run (run.py:1712-1739) over the oracle, with its ledger."""
+def run_oracle(scen, prec: int = 8, threads: int = 8, t_end: float | None = None,
+               h0: np.ndarray | None = None) -> RunResult:  # fmt: skip
+    """The run loop of run._run (run.py:1712-1739) over the oracle, with its ledger; dry or from mean depth h0."""
     tables = scen.build()
     bset = boundary_set(scen.boundaries, tables)
     rain = Rain(tables, bset, scen.cumulative_mm, scen.weight)
     m = OracleModel(tables, ONumerics.like(scen.numerics), prec=prec, threads=threads)
-    m.set_depth(np.zeros(tables.grid.shape))
+    m.set_depth(np.zeros(tables.grid.shape) if h0 is None else h0)
     if bset.size:
         m.set_boundary(bset.cells)
     interior = tables.grid.mask & ~bset.mask
