@@ -156,6 +156,35 @@ int w2d_set_launch(w2d_handle* h, int32_t faces_threads, int32_t cells_threads);
 /* synchronize (gpu/backend.py:202-203). */
 int w2d_synchronize(w2d_handle* h);
 const char* w2d_last_error(void);
+
+/* ---- The learned face closure's network (16 -> 64 -> 64 -> 2, ReLU; BF16 weights on matrix cores, FP32
+ * accumulation). No counterpart in the reference, which has no learned component: these serve the closure's
+ * training driver. Pointers are device pointers; `stream` is a hipStream_t (null: the default stream); nothing
+ * waits. Parameter layouts: FP32 vector W1[64][16], b1[64], W2[64][64], b2[64], W3[2][64], b3[2]; BF16 block W1,
+ * W2, W3 padded to [16][64]. Inputs x are BF16 [n][16], outputs FP32 [n][2]. */
+int w2d_mlp_sizes(int32_t* nparam, int32_t* nbf16);
+/* Plain device memory for the training driver's arrays: allocate, free, copy (kind 0 host to device, 1 device to
+ * host, 2 device to device; synchronous), wait for the device. */
+int w2d_dev_alloc(int64_t bytes, void** out);
+int w2d_dev_free(void* p);
+int w2d_dev_copy(void* dst, const void* src, int64_t bytes, int32_t kind);
+int w2d_dev_synchronize(void);
+/* A minibatch: rows idx[0 .. n) of x [.][16] (BF16), target and mask [.][2], into dense arrays. */
+int w2d_mlp_take(const uint16_t* x, const float* target, const float* mask, const int64_t* idx, int64_t n,
+                 uint16_t* x_out, float* target_out, float* mask_out, void* stream);
+/* BF16 weight block from the FP32 parameters. */
+int w2d_mlp_pack(const float* params, uint16_t* wbf16, void* stream);
+int w2d_mlp_forward(const uint16_t* wbf16, const float* params, const uint16_t* x, int64_t n, float* y, void* stream);
+/* Workgroups w2d_mlp_gradient uses for n samples (its partial buffer holds blocks * (nparam + 1) floats). */
+int w2d_mlp_train_blocks(int64_t n, int32_t* blocks);
+/* Gradient of sum over samples and outputs of mask * (y - target)^2 / norm with respect to the FP32 parameters
+ * (through the BF16 forward pass), into grad[0 .. nparam), and the loss into grad[nparam]. */
+int w2d_mlp_gradient(const uint16_t* wbf16, const float* params, const uint16_t* x, const float* target,
+                     const float* mask, int64_t n, float norm, float* partial, int32_t blocks, float* grad,
+                     void* stream);
+/* One Adam step (step counts from 1) on the FP32 parameters, then their BF16 block. */
+int w2d_mlp_adam(float* params, float* m, float* v, const float* grad, int32_t step, float lr, float beta1,
+                 float beta2, float eps, uint16_t* wbf16, void* stream);
 /* Devices the HIP runtime sees (select_backend's probe, solver/backend.py:58-75). */
 int w2d_device_count(int32_t* out);
 

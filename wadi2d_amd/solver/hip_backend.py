@@ -77,6 +77,17 @@ _SIGS = {
     "w2d_storage": [VP, ctypes.POINTER(D)],
     "w2d_synchronize": [VP],
     "w2d_set_launch": [VP, I32, I32],
+    "w2d_mlp_sizes": [ctypes.POINTER(I32), ctypes.POINTER(I32)],
+    "w2d_dev_alloc": [I64, ctypes.POINTER(VP)],
+    "w2d_dev_free": [VP],
+    "w2d_dev_copy": [VP, VP, I64, I32],
+    "w2d_dev_synchronize": [],
+    "w2d_mlp_take": [VP, VP, VP, VP, I64, VP, VP, VP, VP],
+    "w2d_mlp_pack": [VP, VP, VP],
+    "w2d_mlp_forward": [VP, VP, VP, I64, VP, VP],
+    "w2d_mlp_train_blocks": [I64, ctypes.POINTER(I32)],
+    "w2d_mlp_gradient": [VP, VP, VP, VP, VP, I64, ctypes.c_float, VP, I32, VP, VP],
+    "w2d_mlp_adam": [VP, VP, VP, VP, I32, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float, VP, VP],
     "w2d_source_weights": [VP, VP, VP, I64],
     "w2d_source_scale": [VP, D],
     "w2d_source_sum": [VP, ctypes.POINTER(D)],
