@@ -149,6 +149,14 @@ int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
 int w2d_run_idle(w2d_handle* h, int32_t* out);
 /* The active list as the device last left it (out may be null to ask for the count). */
 int w2d_active_tiles(w2d_handle* h, int32_t* out, int32_t* n);
+/* The learned face closure (no counterpart in the reference, which has none): n faces, each the west (axis 1) or
+ * north (axis 0) face of own-layout cell cell_r[f] (t * 1024 + j * 32 + i) with cell_l[f] across it; stat[f][11]
+ * their static features; norm the 16 feature means, 16 inverse deviations and the scale on the invert-rise output;
+ * params the network's FP32
+ * parameters (w2d_mlp_*). Before every step the network corrects those faces from the current levels: their
+ * inverts rise and, with `conveyance`, their K^(3/5) scales. FP32 handles only; n = 0 turns it off. */
+int w2d_closure_set(w2d_handle* h, int32_t n, const int32_t* cell_r, const int32_t* cell_l, const int32_t* axis,
+                    const float* stat, const float* norm, const float* params, int32_t conveyance);
 /* Workgroup sizes of the face and cell kernels, 256 or 512 threads each (defaults 512, 256); the fields do not
  * depend on them. No counterpart in the reference, whose launch shape is fixed (gpu/backend.py:151-188): the
  * launch-shape tuner's knob. */

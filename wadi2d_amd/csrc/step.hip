@@ -30,8 +30,10 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
+#include "mlp.hpp"
 #include "wadi2d_hip.h"
 
 #pragma clang fp contract(off)
@@ -432,16 +434,25 @@ struct FaceIn {
 };
 
 // One face: the west (x) or north (y) face of own cell L of the LDS block; faces.cu:105-242 restated, with the
-// statement order of solver/cpu/kernels.py:86-219. Returns the wave speed.
-template <class R, bool FAST>
+// statement order of solver/cpu/kernels.py:86-219. Returns the wave speed. LEARNED: the learned closure's
+// correction of this face, `corr` = (conveyance factor on T, rise of the face's inverts), is applied (no counterpart
+// in the reference; the plain path never takes this branch).
+template <class R, bool FAST, bool LEARNED = false>
 __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st, R ef, R cw, const float* zc,
                                   const float4* __restrict__ ftab, size_t plane, int last, float4* cache,
                                   unsigned char* tag, const R* __restrict__ om, unsigned k_up, unsigned k_own, R dt,
-                                  bool want_wave, const Params& P, R& u_out, R& m_out, R& q_out) {
+                                  bool want_wave, const Params& P, R& u_out, R& m_out, R& q_out,
+                                  float2 corr = float2{1.f, 0.f}) {
     const int S = in.S, Q = in.P;
     const R* u = in.su;
     const R g = R(P.g), dx = R(P.dx);
-    const R fl = R(st.x), fr = R(st.y), gl = R(st.z), gr = R(st.w);
+    R fl = R(st.x), fr = R(st.y), gl = R(st.z), gr = R(st.w);
+    if constexpr (LEARNED) {
+        fl += R(corr.y);
+        fr += R(corr.y);
+        gl += R(corr.y);
+        gr += R(corr.y);
+    }
     const R eL = in.sd[L - S] - fl;
     const R eR = in.sd[L] - fr;
     const R uf = u[L];
@@ -467,6 +478,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     }
     R A, T, W;
     face_lookup<R, FAST>(eface, ef, ftab, plane, last, A, T, W, cache, tag, have);
+    if constexpr (LEARNED) T *= R(corr.x);
     const R hL = in.sh[L - S];
     const R hR = in.sh[L];
     const R hup = from_left ? hL : hR;
@@ -633,10 +645,13 @@ struct StepArgs {
     const RunGlobals* glob;
     const double* src_weight;
     double* src_partial;
+    // the learned closure's per-face corrections (null when it is off)
+    const float2* corr_x;
+    const float2* corr_y;
 };
 
 // Both own faces of every own cell of one active tile per workgroup of NTH threads (faces.cu:244-271).
-template <class R, bool FAST, int NTH>
+template <class R, bool FAST, int NTH, bool LEARNED = false>
 __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     __shared__ __align__(16) R s_u[LSIZE];
     __shared__ __align__(16) R s_v[LSIZE];
@@ -679,15 +694,20 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
         const float4 e = A.st.ec[c];
         R un, mn, qn;
-        const R wx = face<R, FAST>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
-                                   A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c, A.s.om,
-                                   west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn);
+        float2 cx{1.f, 0.f}, cy{1.f, 0.f};
+        if constexpr (LEARNED) {
+            cx = A.corr_x[c];
+            cy = A.corr_y[c];
+        }
+        const R wx = face<R, FAST, LEARNED>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
+                                            A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
+                                            A.s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
         A.s.u1[c] = un;
         A.s.mx1[c] = mn;
         A.s.qx1[c] = qn;
-        const R wy = face<R, FAST>(fy, L, A.s.my[c], A.st.sy[c], R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
-                                   A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c, A.s.om,
-                                   north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn);
+        const R wy = face<R, FAST, LEARNED>(fy, L, A.s.my[c], A.st.sy[c], R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
+                                            A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
+                                            A.s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
         A.s.v1[c] = un;
         A.s.my1[c] = mn;
         A.s.qy1[c] = qn;
@@ -697,6 +717,74 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     if (want_wave) {
         const R m = block_max<NTH>(w, red);
         if (threadIdx.x == 0) A.wave[t] = m;
+    }
+}
+
+// The learned closure on its faces, before a step (no counterpart in the reference): each wave takes 16 faces, forms
+// their 16 features (11 static, then e_L, e_R, h_L, h_R and the upwind level over the crest, from the current level
+// and depth), standardises them, evaluates the network on the matrix cores and leaves the face's correction:
+// conveyance factor exp(clamp(y0)) (1 unless `conveyance`) and invert rise clamp(y1 * scale, 0, 4 m).
+struct ClosureArgs {
+    int n;
+    const int* cell_r;  // the face's own cell (whose west or north face it is), own layout
+    const int* cell_l;  // the cell on its other side
+    const int* axis;    // 1: x face, 0: y face
+    const float* stat;  // n x 11
+    const float* norm;  // 16 means, 16 inverse deviations, then the scale on the invert-rise output
+    const uint16_t* wbf;
+    const float* params;
+    const float4* sx;
+    const float4* sy;
+    float2* corr_x;
+    float2* corr_y;
+    int conveyance;
+};
+
+constexpr int CLOSURE_STATIC = 11;
+constexpr float CLOSURE_LOGC = 1.5f, CLOSURE_RISE = 4.f;
+
+__global__ __launch_bounds__(256) void closure_kernel(ClosureArgs C, const float* __restrict__ d,
+                                                      const float* __restrict__ h) {
+    using namespace w2d_mlp;
+    __shared__ Net net;
+    __shared__ __align__(16) uint16_t xs[4][WAVE_SAMPLES * IN];
+    __shared__ __align__(16) uint16_t h1[4][WAVE_SAMPLES * HID];
+    __shared__ __align__(16) uint16_t h2[4][WAVE_SAMPLES * HID];
+    __shared__ float ys[4][WAVE_SAMPLES * OUT];
+    load_net(net, C.wbf, C.params);
+    __syncthreads();
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * 64; base < C.n; base += gridDim.x * 64) {
+        const int f = base + wv * WAVE_SAMPLES + (lane & 15);
+        if (lane < 16) {
+            float x[IN];
+            if (f < C.n) {
+                const int r = C.cell_r[f], l = C.cell_l[f];
+                const float4 st = C.axis[f] ? C.sx[r] : C.sy[r];
+                const float* s = C.stat + (size_t)f * CLOSURE_STATIC;
+                for (int q = 0; q < CLOSURE_STATIC; ++q) x[q] = s[q];
+                const float eL = fmaxf(d[l] - st.x, 0.f), eR = fmaxf(d[r] - st.y, 0.f);
+                x[11] = eL;
+                x[12] = eR;
+                x[13] = h[l];
+                x[14] = h[r];
+                x[15] = fmaxf(eL, eR) - s[5];
+            } else {
+                for (int q = 0; q < IN; ++q) x[q] = C.norm[q];
+            }
+            for (int q = 0; q < IN; ++q) xs[wv][lane * IN + q] = to_bf16((x[q] - C.norm[q]) * C.norm[IN + q]);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        forward_wave(net, xs[wv], h1[wv], h2[wv], ys[wv]);
+        if (lane < 16 && f < C.n) {
+            const float y0 = ys[wv][lane * OUT], y1 = ys[wv][lane * OUT + 1];
+            const float c = C.conveyance ? expf(fminf(fmaxf(y0, -CLOSURE_LOGC), CLOSURE_LOGC)) : 1.f;
+            const float rise = fminf(fmaxf(y1 * C.norm[2 * IN], 0.f), CLOSURE_RISE);
+            (C.axis[f] ? C.corr_x : C.corr_y)[C.cell_r[f]] = make_float2(c, rise);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1314,6 +1402,12 @@ __global__ void fill_kernel(R* a, size_t n, R v) {
 // host side
 
 struct w2d_handle {
+    // the learned closure (off while closure_n is 0)
+    int closure_n = 0, closure_conveyance = 0;
+    int *cl_cell_r = nullptr, *cl_cell_l = nullptr, *cl_axis = nullptr;
+    float *cl_stat = nullptr, *cl_norm = nullptr, *cl_params = nullptr;
+    uint16_t* cl_wbf = nullptr;
+    float2 *corr_x = nullptr, *corr_y = nullptr;
     int prec = 4;
     int fast = 0;
     int face_cache = 1;
@@ -1462,6 +1556,8 @@ StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     A.st.zcy = H->zcy;
     A.st.fcx = H->face_cache ? H->fcx : nullptr;
     A.st.fcy = H->face_cache ? H->fcy : nullptr;
+    A.corr_x = H->closure_n ? H->corr_x : nullptr;
+    A.corr_y = H->closure_n ? H->corr_y : nullptr;
     A.st.ftagx = H->ftagx;
     A.st.ftagy = H->ftagy;
     A.st.cc = H->cc;
@@ -1497,6 +1593,20 @@ int mark(w2d_handle* H) {
 // same LDS per workgroup, twice the waves per CU).
 template <class R, bool FAST>
 int launch_faces(w2d_handle* H, const StepArgs<R>& A, int blocks) {
+    if (H->closure_n) {
+        if constexpr (std::is_same<R, float>::value) {
+            ClosureArgs C{H->closure_n, H->cl_cell_r, H->cl_cell_l, H->cl_axis, H->cl_stat, H->cl_norm, H->cl_wbf,
+                          H->cl_params, H->sx, H->sy, H->corr_x, H->corr_y, H->closure_conveyance};
+            const int cb = std::min((H->closure_n + 63) / 64, 4096);
+            closure_kernel<<<cb, 256, 0, H->stream>>>(C, A.s.d, A.s.h);
+            HIP_TRY(hipGetLastError());
+            faces_kernel<float, FAST, 512, true><<<blocks, 512, 0, H->stream>>>(A);
+            HIP_TRY(hipGetLastError());
+            return 0;
+        } else {
+            return fail(1, "the learned closure runs in FP32");
+        }
+    }
     if (H->faces_threads == 512)
         faces_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
     else
@@ -2119,6 +2229,63 @@ int w2d_copy_probe(int64_t bytes, int32_t repeats, double* ms) {
     (void)hipEventDestroy(e1);
     (void)hipFree(a);
     (void)hipFree(b);
+    return 0;
+}
+
+int w2d_closure_set(w2d_handle* H, int32_t n, const int32_t* cell_r, const int32_t* cell_l, const int32_t* axis,
+                    const float* stat, const float* norm, const float* params, int32_t conveyance) {
+    if (!H) return fail(1, "closure_set: no handle");
+    if (n < 0 || (n && (!cell_r || !cell_l || !axis || !stat || !norm || !params)))
+        return fail(1, "closure_set: bad arguments");
+    if (n && H->prec != 4) return fail(1, "closure_set: the learned closure runs in FP32");
+    for (int f = 0; f < n; ++f)
+        if (cell_r[f] < (int)TILE2 || (size_t)cell_r[f] >= H->ncell || cell_l[f] < (int)TILE2 ||
+            (size_t)cell_l[f] >= H->ncell || (axis[f] != 0 && axis[f] != 1))
+            return fail(1, "closure_set: a face is out of range");
+    auto drop = [&](void* p) {
+        if (!p) return;
+        H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), p), H->owned.end());
+        (void)hipFree(p);
+    };
+    drop(H->cl_cell_r);
+    drop(H->cl_cell_l);
+    drop(H->cl_axis);
+    drop(H->cl_stat);
+    drop(H->cl_norm);
+    drop(H->cl_params);
+    drop(H->cl_wbf);
+    H->cl_cell_r = H->cl_cell_l = H->cl_axis = nullptr;
+    H->cl_stat = H->cl_norm = H->cl_params = nullptr;
+    H->cl_wbf = nullptr;
+    if (H->graph) {  // a captured graph holds the old launches
+        (void)hipGraphExecDestroy(H->graph);
+        H->graph = nullptr;
+        H->graph_steps = 0;
+    }
+    H->closure_n = 0;
+    if (n == 0) return 0;
+    int s;
+    using namespace w2d_mlp;
+    std::vector<uint16_t> wbf(NBF16, 0);
+    for (int e = 0; e < HID * IN; ++e) wbf[BF_W1 + e] = to_bf16(params[OFF_W1 + e]);
+    for (int e = 0; e < HID * HID; ++e) wbf[BF_W2 + e] = to_bf16(params[OFF_W2 + e]);
+    for (int e = 0; e < OUT * HID; ++e) wbf[BF_W3 + e] = to_bf16(params[OFF_W3 + e]);
+    if ((s = upload(H, &H->cl_cell_r, cell_r, n)) || (s = upload(H, &H->cl_cell_l, cell_l, n)) ||
+        (s = upload(H, &H->cl_axis, axis, n)) || (s = upload(H, &H->cl_stat, stat, (size_t)n * CLOSURE_STATIC)) ||
+        (s = upload(H, &H->cl_norm, norm, 2 * IN + 1)) || (s = upload(H, &H->cl_params, params, NPARAM)) ||
+        (s = upload(H, &H->cl_wbf, wbf.data(), NBF16)))
+        return s;
+    // every face starts uncorrected: a face left out of this set keeps no correction from an earlier one
+    std::vector<float2> one(H->ncell, make_float2(1.f, 0.f));
+    if (!H->corr_x) {
+        if ((s = upload(H, &H->corr_x, one.data(), one.size())) || (s = upload(H, &H->corr_y, one.data(), one.size())))
+            return s;
+    } else {
+        HIP_TRY(hipMemcpy(H->corr_x, one.data(), one.size() * sizeof(float2), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(H->corr_y, one.data(), one.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    H->closure_n = n;
+    H->closure_conveyance = conveyance != 0;
     return 0;
 }
 

@@ -108,15 +108,15 @@ def stack_tables(members: list[GridTables]) -> tuple[GridTables, list[Slot]]:
 
 
 class MemberRain:
-    """A member's storm over its interior cells (state indices of the mosaic): the window scale the device forms
-    rates from, as ``Rain.scale``."""
+    """A member's storm (cumulative depth in mm against time in s) over its interior cells (state indices of the
+    mosaic): the window scale the device forms rates from, as ``Rain.scale``."""
 
-    def __init__(self, storm: ScsStorm, state: np.ndarray) -> None:
+    def __init__(self, storm, state: np.ndarray) -> None:
         self.storm = storm
         self.state = state
 
     def scale(self, t0: float, t1: float) -> float:
-        return self.storm.mean_rate(t0, t1)
+        return (float(self.storm(t1)) - float(self.storm(t0))) / (1000.0 * (t1 - t0))
 
 
 @dataclass
@@ -131,11 +131,17 @@ class Ensemble:
     @classmethod
     def build(cls, specs: list[MemberSpec], dx: float = 20.0, k: int = 4, duration_s: float = 3 * 3600.0,
               workers: int | None = None) -> Ensemble:  # fmt: skip
-        scens = [member_scenario(s, dx, k, duration_s) for s in specs]
+        return cls.from_scenarios([member_scenario(s, dx, k, duration_s) for s in specs], specs, workers)
+
+    @classmethod
+    def from_scenarios(cls, scens: list, specs: list | None = None, workers: int | None = None,
+                       keep_beds: bool = False) -> Ensemble:  # fmt: skip
+        """Members from built-or-not scenarios (their storms are their ``cumulative_mm``)."""
         built = []
         for sc in scens:
             built.append(sc.build(workers))
-            sc.bed = None  # the pixels are not needed again
+            if not keep_beds:
+                sc.bed = None  # the pixels are not needed again
         tables, slots = stack_tables(built)
         bounds = []
         for sc, sl in zip(scens, slots, strict=True):
@@ -152,6 +158,9 @@ class Ensemble:
             ens.interior.append(np.asarray(state_index(tables.grid, jj, ii), np.int64))
         ens.bset = bset
         return ens
+
+    def storm(self, m: int):
+        return self.scenarios[m].cumulative_mm
 
     @property
     def cells(self) -> int:
@@ -182,7 +191,8 @@ class BatchedResult:
 
 
 def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chunk: int = 512, tune: bool = True,
-                launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None) -> BatchedResult:  # fmt: skip
+                launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None, on_chunk=None,
+                prepare=None) -> BatchedResult:  # fmt: skip
     """Every member in one device-resident run to its end; per-member max depth (sampled every ``maxima_every``
     steps), mass error."""
     from wadi2d_amd.engine.autotune import tune_launch
@@ -205,9 +215,11 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
     st = model.stepper
     plans = []
     for m, sl in enumerate(ens.slots):
-        rain = MemberRain(ens.specs[m].storm, ens.interior[m])
+        rain = MemberRain(ens.storm(m), ens.interior[m])
         plans.append(MemberPlan(sl.tile0, sl.tile1, rain, None, 0.0, ens.scenarios[m].duration_s, st.dt, 0, True,
                                 ens.interior[m]))  # fmt: skip
+    if prepare is not None:
+        prepare(model)
     run = DeviceRun(b, plans, num, maxima=maxima_every, idle=st.idle)
     b.synchronize()
     t0 = time.perf_counter()
@@ -219,6 +231,8 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
     state = run.state
     while not all(s.done for s in state):
         state = run.steps(chunk, graph_steps)
+        if on_chunk is not None:
+            on_chunk(model, state)
         if log is not None:
             live = sum(not s.done for s in state)
             log(f"[ensemble] {backend}: {max(s.nsteps for s in state)} steps, {live} members live, "
@@ -249,13 +263,13 @@ def run_serial(ens: Ensemble, backend: str = "hip", members: list[int] | None = 
                log=None) -> SerialResult:  # fmt: skip
     """The plain way: each member its own model, driven from the host one step at a time, one member after another
     (a storm study's serial loop, arf.py:347-371)."""
-    which = range(len(ens.specs)) if members is None else members
+    which = range(len(ens.scenarios)) if members is None else members
     wall, steps, dmaxes, masses = 0.0, [], [], []
     for m in which:
         sc = ens.scenarios[m]
         tables = sc.tables
         bset = boundary_set(sc.boundaries, tables)
-        rain = Rain(tables, bset, ens.specs[m].storm)
+        rain = Rain(tables, bset, ens.storm(m))
         t0 = time.perf_counter()
         model = Model(tables, sc.numerics, backend)
         model.set_depth(np.zeros(tables.grid.shape))
@@ -274,5 +288,5 @@ def run_serial(ens: Ensemble, backend: str = "hip", members: list[int] | None = 
 
 def max_depth_error(a: list[np.ndarray], b: list[np.ndarray], ens: Ensemble, members=None) -> list[float]:
     """Per member, the largest |difference| of max depth over the member's cells."""
-    which = range(len(ens.specs)) if members is None else members
+    which = range(len(ens.scenarios)) if members is None else members
     return [float(np.abs(x - y)[ens.slots[m].grid.mask].max()) for m, x, y in zip(which, a, b, strict=True)]

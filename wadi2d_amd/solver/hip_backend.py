@@ -77,6 +77,7 @@ _SIGS = {
     "w2d_storage": [VP, ctypes.POINTER(D)],
     "w2d_synchronize": [VP],
     "w2d_set_launch": [VP, I32, I32],
+    "w2d_closure_set": [VP, I32, VP, VP, VP, VP, VP, VP, I32],
     "w2d_mlp_sizes": [ctypes.POINTER(I32), ctypes.POINTER(I32)],
     "w2d_dev_alloc": [I64, ctypes.POINTER(VP)],
     "w2d_dev_free": [VP],
@@ -252,6 +253,13 @@ class HipBackend:
 
     def synchronize(self) -> None:
         check(api().w2d_synchronize(self.h))
+
+    def closure_set(self, cell_r, cell_l, axis, stat, norm, params, conveyance: bool = True) -> None:
+        """The learned closure on the listed faces (own-layout cell indices); an empty list turns it off."""
+        arrs = [_c(cell_r, np.int32), _c(cell_l, np.int32), _c(axis, np.int32), _c(stat, np.float32),
+                _c(norm, np.float32), _c(params, np.float32)]  # fmt: skip
+        n = int(arrs[0].size)
+        check(api().w2d_closure_set(self.h, n, *[a.ctypes.data for a in arrs], int(conveyance)))
 
     def set_launch(self, faces_threads: int = 512, cells_threads: int = 256) -> None:
         """Workgroup sizes of the face and cell kernels (256 or 512 each); the fields do not depend on them."""

@@ -218,6 +218,41 @@ def precision_case(n: int = 2048, seed: int = 3, k: int = 5, duration_s: float =
     return scen
 
 
+def crest_case(n: int, dx: float, k: int, seed: int, storm, duration_s: float, extent: float | None = None,
+               name: str = "crested", density: float = 10.0, heights=(0.3, 1.5)) -> Scenario:
+    """Ground defined in world coordinates over a square of side ``extent`` (default n dx), sampled on any grid of
+    n x n cells (so a fine and a coarse grid of one seed see the same ground): a plane falling 0.5 % east, 6-octave
+    value noise (2 m, base 640 m), and ``density`` crest ridges per km2, straight and flat-topped, 100-600 m long,
+    10-80 m wide (0.5-4 cells of 20 m), ``heights`` high (uniform), at seeded places and angles. 8 rows, n 0.04, a
+    free east column."""
+    extent = n * dx if extent is None else extent
+    grid = TileGrid.from_mask(np.ones((n, n), bool), 0.0, extent, dx)
+    x, y, p = _pixel_centres(grid, k)
+    z = np.zeros((y.size, x.size))
+    noise(z, grid.x0, grid.y0, p, seed, 6, 640.0, 0.5, 2.0)
+    z += 0.005 * (extent - x)[None, :]
+    rng = np.random.default_rng(seed)
+    count = int(round(density * (extent / 1000.0) ** 2))
+    for _ in range(count):
+        c = rng.uniform(0.1, 0.9, 2) * extent
+        ang = rng.uniform(0.0, np.pi)
+        half = 0.5 * rng.uniform(100.0, 600.0) * np.array([np.cos(ang), np.sin(ang)])
+        width = rng.uniform(0.5, 4.0) * 20.0
+        height = rng.uniform(*heights)
+        a, b = c - half, c + half
+        lo, hi = np.minimum(a, b) - width, np.maximum(a, b) + width
+        ci = np.nonzero((x >= lo[0]) & (x <= hi[0]))[0]
+        rj = np.nonzero((y >= lo[1]) & (y <= hi[1]))[0]
+        if ci.size == 0 or rj.size == 0:
+            continue
+        dist = _segments_distance(x[ci][None, :], y[rj][:, None], a, b)
+        block = z[np.ix_(rj, ci)]
+        block[dist <= 0.5 * width] += height
+        z[np.ix_(rj, ci)] = block
+    east = [Boundary("free", np.arange(1, n + 1), np.full(n, n))]
+    return Scenario(name, grid, ArrayBed(z, p), Tables(rows=8), 0.04, east, storm, None, duration_s, Numerics())
+
+
 def reservoir_depth(scen: Scenario, above: float = 5.0) -> np.ndarray:
     """Mean depth of every reservoir cell with its water surface ``above`` metres over the cell's mean ground (the
     level read through the cell table, tables/builder.py:311-331), zero elsewhere."""
