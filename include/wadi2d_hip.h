@@ -153,8 +153,9 @@ int w2d_active_tiles(w2d_handle* h, int32_t* out, int32_t* n);
  * north (axis 0) face of own-layout cell cell_r[f] (t * 1024 + j * 32 + i) with cell_l[f] across it; stat[f][11]
  * their static features; norm the 16 feature means, 16 inverse deviations and the scale on the invert-rise output;
  * params the network's FP32
- * parameters (w2d_mlp_*). Before every step the network corrects those faces from the current levels: their
- * inverts rise and, with `conveyance`, their K^(3/5) scales. FP32 handles only; n = 0 turns it off. */
+ * parameters (w2d_mlp_*). Before every step that refreshes the wave speed (and so before the first) the network
+ * corrects those faces from the current levels: their inverts rise and, with `conveyance`, their K^(3/5) scales.
+ * FP32 handles only; n = 0 turns it off. */
 int w2d_closure_set(w2d_handle* h, int32_t n, const int32_t* cell_r, const int32_t* cell_l, const int32_t* axis,
                     const float* stat, const float* norm, const float* params, int32_t conveyance);
 /* Workgroup sizes of the face and cell kernels, 256 or 512 threads each (defaults 512, 256); the fields do not

@@ -97,3 +97,24 @@ def test_a_closure_that_raises_a_line_of_faces_holds_the_water():
     print("water east of the line:", float(runs[0.0][east].sum()), float(runs[4.0][east].sum()), flush=True)
     assert runs[0.0][east].sum() > 1.0
     assert runs[4.0][east].sum() == 0.0
+
+
+@pytest.mark.gpu
+def test_the_trained_closure_beats_the_plain_table_path_on_unseen_ground():
+    """The committed network on four members it never saw (seeds apart from training and calibration): their 20 m
+    runs with and without it against their own 2 m runs coarse-grained. The plain table path is the judge: the
+    learned run must cut its max-depth RMSE by at least 15 %."""
+    from wadi2d_amd.closure.data import fine_runs
+    from wadi2d_amd.closure.evaluate import coarse_run, errors
+    from wadi2d_amd.closure.model import ClosureModel
+
+    model = ClosureModel.load("wadi2d_amd/closure/trained.npz")
+    col, pairs = fine_runs([7001, 7002, 7003, 7004], 64, 5400.0, chunk=64, storm_seed=7)
+    coarse = [c for _, c in pairs]
+    plain, _ = coarse_run(coarse, col.coarse, None)
+    learned, _ = coarse_run(coarse, col.coarse, model)
+    ep = errors(plain.max_depth, col.dmax, col.coarse)
+    el = errors(learned.max_depth, col.dmax, col.coarse)
+    print(f"max depth against the 2 m runs: plain rmse {ep.rmse:.4f} max {ep.max_abs:.3f}; learned rmse "
+          f"{el.rmse:.4f} max {el.max_abs:.3f} ({ep.cells} cells)", flush=True)
+    assert el.rmse <= 0.85 * ep.rmse

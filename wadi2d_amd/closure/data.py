@@ -92,10 +92,9 @@ def face_T(tables, axis: int, jj: np.ndarray, ii: np.ndarray, e: np.ndarray) -> 
     return out
 
 
-def coarse_grain(h: np.ndarray, qx: np.ndarray, qy: np.ndarray, n: int):
-    """Fine domain fields (n RATIO square) -> coarse mean depth and face discharges on the padded coarse grid
-    (n + 2 rounded up to whole tiles): hbar[j, i], Qx at the west face of (j, i), Qy at its north face."""
-    r = RATIO
+def coarse_grain(h: np.ndarray, qx: np.ndarray, qy: np.ndarray, n: int, r: int = RATIO):
+    """Fine domain fields (n r square) -> coarse domain mean depth (n x n) and face discharges: Qx at the west face
+    of (j, i), Qy at its north face."""
     nf = n * r
     hb = h.reshape(n, r, n, r).mean(axis=(1, 3))
     Qx = np.zeros((n, n))
@@ -106,12 +105,15 @@ def coarse_grain(h: np.ndarray, qx: np.ndarray, qy: np.ndarray, n: int):
 
 
 class Collector:
-    """Coarse-grains snapshots of fine members into samples, and keeps each member's coarse-grained max level."""
+    """Coarse-grains snapshots of fine members (``ratio`` fine cells per coarse cell side) into samples, and keeps
+    each member's coarse-grained max level."""
 
-    def __init__(self, ens, pairs: list, n: int) -> None:
+    def __init__(self, ens, pairs: list, n: int, ratio: int = RATIO, samples: bool = True, coarse=None) -> None:
         self.ens = ens
         self.n = n
-        self.coarse = [CoarseFaces.of(c) for _, c in pairs]
+        self.ratio = ratio
+        self.keep_samples = samples
+        self.coarse = coarse if coarse is not None else [CoarseFaces.of(c) for _, c in pairs]
         self.samples: list[tuple[np.ndarray, np.ndarray, np.ndarray]] = []
         self.dmax = [np.zeros(cf.hc.shape) for cf in self.coarse]
         self.snapshots = 0
@@ -120,15 +122,17 @@ class Collector:
         b = model.backend
         g = self.ens.tables.grid
         fields = {f: from_state(g, b.get_field(f)).astype(np.float64) for f in ("h", "qx", "qy")}
-        n, nf = self.n, self.n * RATIO
+        n, nf = self.n, self.n * self.ratio
         for m, cf in enumerate(self.coarse):
             block = {f: self.ens.member_field(m, a)[1 : nf + 1, 1 : nf + 1] for f, a in fields.items()}
-            hb, Qx, Qy = coarse_grain(block["h"], block["qx"], block["qy"], n)
+            hb, Qx, Qy = coarse_grain(block["h"], block["qx"], block["qy"], n, self.ratio)
             shape = cf.hc.shape
             hbar = np.zeros(shape)
             hbar[1 : n + 1, 1 : n + 1] = hb
             d = level_of(hbar, cf.hc, cf.dtab)
             self.dmax[m] = np.maximum(self.dmax[m], d)
+            if not self.keep_samples:
+                continue
             Q = {1: np.zeros(shape), 0: np.zeros(shape)}
             Q[1][1 : n + 1, 1 : n + 1] = Qx
             Q[0][1 : n + 1, 1 : n + 1] = Qy
@@ -170,6 +174,24 @@ class Collector:
         if not self.samples:
             return np.zeros((0, NSTATIC + 5), np.float32), np.zeros((0, 2), np.float32), np.zeros((0, 2), np.float32)
         return tuple(np.concatenate(a) for a in zip(*self.samples, strict=True))
+
+
+def middle_runs(pairs: list, coarse: list, n: int, ratio: int, duration_s: float, chunk: int = 64, log=None):
+    """The members of ``pairs`` run the plain way on a grid ``ratio`` times finer than the coarse one (same
+    pixels, 20 m / ratio cells), coarse-grained onto the coarse grid as the fine runs are. Returns the collector
+    and the run (its wall time)."""
+    from wadi2d_amd.engine.ensemble import Ensemble, run_batched
+
+    scens = []
+    for (fine, c), cf in zip(pairs, coarse, strict=True):
+        seed = int(c.name.split()[-1])
+        scens.append(synth.crest_case(n * ratio, COARSE_DX / ratio, RATIO // ratio, seed, c.cumulative_mm,
+                                      duration_s, n * COARSE_DX, f"middle {seed}"))  # fmt: skip
+    ens = Ensemble.from_scenarios(scens)
+    col = Collector(ens, None, n, ratio, samples=False, coarse=coarse)
+    run = run_batched(ens, "hip", graph_steps=64, chunk=chunk, tune=False, launch=(512, 256), maxima_every=0, log=log,
+                      on_chunk=col)  # fmt: skip
+    return col, run
 
 
 def fine_runs(seeds: list[int], n: int, duration_s: float, chunk: int = 256, log=None, storm_seed: int = 0):

@@ -64,27 +64,36 @@ def face_statics(tables, z: np.ndarray, k: int, axis: int) -> dict:
     valid = np.zeros_like(mask)
     valid[:, 1:] = mask[:, 1:] & mask[:, :-1]
     valid &= fl < 1e8  # closed faces carry the CLOSED invert
-    rel = prof - inv[:, None, :]
-    pct = np.nanpercentile(np.where(valid[:, None, :], rel, 0.0), [10, 25, 50, 75, 90], axis=1)
-    ph = P - inv[:, None, :]
-    crest_h = np.median(np.where(valid[:, None, :], ph, 0.0), axis=1)
-    sill = np.min(np.where(valid[:, None, :], ph, 0.0), axis=1)
+    # order statistics over the k pixel rows of each face, rows last and sorted (linear interpolation as
+    # numpy's default percentile)
+    rel = np.sort(np.moveaxis(prof - inv[:, None, :], 1, -1), axis=-1)
+    ph_sorted = np.sort(np.moveaxis(P - inv[:, None, :], 1, -1), axis=-1)
+
+    def pct(a, q):
+        pos = q / 100.0 * (k - 1)
+        lo = int(np.floor(pos))
+        hi = min(lo + 1, k - 1)
+        return a[..., lo] + (pos - lo) * (a[..., hi] - a[..., lo])
+
+    pcts = [pct(rel, q) for q in (10, 25, 50, 75, 90)]
+    crest_h = pct(ph_sorted, 50)
+    sill = ph_sorted[..., 0]
     # crest width: path pixels above half the row's crest height, in metres
     pixel = grid.dx / k
-    half = inv[:, None, :] + 0.5 * np.where(valid[:, None, :], np.maximum(ph, 0.0), 0.0)  # (nyy, k, nxx)
+    ph = P - inv[:, None, :]
+    half = inv[:, None, :] + 0.5 * np.maximum(np.nan_to_num(ph, nan=0.0), 0.0)  # (nyy, k, nxx)
     zr = zz.reshape(nyy, k, nxx, k)
     h2 = k // 2
     count = np.zeros((nyy, k, nxx))
     count[:, :, 1:] = ((zr[:, :, :-1, h2:] > half[:, :, 1:, None]).sum(-1) +
                        (zr[:, :, 1:, :h2] > half[:, :, 1:, None]).sum(-1))  # fmt: skip
-    tall = np.where(valid[:, None, :], ph, 0.0) > 0.05
+    tall = np.nan_to_num(ph, nan=0.0) > 0.05
     width = np.where(tall, count, 0.0).mean(axis=1) * pixel
     dzbar = np.zeros_like(zbar)
     dzbar[:, 1:] = zbar[:, 1:] - zbar[:, :-1]
-    feats = np.stack([pct[0], pct[1], pct[2], pct[3], pct[4], crest_h, width, ef, dzbar, (cw != 0).astype(float),
-                      np.ones_like(ef)], axis=-1)  # fmt: skip
-    feats = np.where(valid[..., None], feats, 0.0)
-    sill = np.where(valid, np.maximum(sill, 0.0), 0.0)
+    feats = np.stack([*pcts, crest_h, width, ef, dzbar, (cw != 0).astype(float), np.ones_like(ef)], axis=-1)
+    feats = np.where(valid[..., None], np.nan_to_num(feats, nan=0.0), 0.0)
+    sill = np.where(valid, np.maximum(np.nan_to_num(sill, nan=0.0), 0.0), 0.0)
     if axis == 0:
         feats, sill, valid, inv = feats.transpose(1, 0, 2), sill.T, valid.T, inv.T
     return {"static": feats, "sill": sill, "valid": valid, "invert": inv}

@@ -720,7 +720,8 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     }
 }
 
-// The learned closure on its faces, before a step (no counterpart in the reference): each wave takes 16 faces, forms
+// The learned closure on its faces, before a step that refreshes the wave speed (every dt_every steps and after a
+// source change; levels move little between): each wave takes 16 faces, forms
 // their 16 features (11 static, then e_L, e_R, h_L, h_R and the upwind level over the crest, from the current level
 // and depth), standardises them, evaluates the network on the matrix cores and leaves the face's correction:
 // conveyance factor exp(clamp(y0)) (1 unless `conveyance`) and invert rise clamp(y1 * scale, 0, 4 m).
@@ -738,6 +739,9 @@ struct ClosureArgs {
     float2* corr_x;
     float2* corr_y;
     int conveyance;
+    // device-resident run: only the faces of members on a refresh step are evaluated (null: every face)
+    const Member* members;
+    const int* tile_member;
 };
 
 constexpr int CLOSURE_STATIC = 11;
@@ -756,6 +760,14 @@ __global__ __launch_bounds__(256) void closure_kernel(ClosureArgs C, const float
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int base = blockIdx.x * 64; base < C.n; base += gridDim.x * 64) {
         const int f = base + wv * WAVE_SAMPLES + (lane & 15);
+        if (C.members) {  // a wave whose faces all belong to members not refreshing this step has nothing to do
+            bool due = false;
+            if (lane < 16 && f < C.n) {
+                const Member& mb = C.members[C.tile_member[C.cell_r[f] / TILE2]];
+                due = mb.stepping && mb.refresh;
+            }
+            if (!__any(due)) continue;
+        }
         if (lane < 16) {
             float x[IN];
             if (f < C.n) {
@@ -1596,10 +1608,13 @@ int launch_faces(w2d_handle* H, const StepArgs<R>& A, int blocks) {
     if (H->closure_n) {
         if constexpr (std::is_same<R, float>::value) {
             ClosureArgs C{H->closure_n, H->cl_cell_r, H->cl_cell_l, H->cl_axis, H->cl_stat, H->cl_norm, H->cl_wbf,
-                          H->cl_params, H->sx, H->sy, H->corr_x, H->corr_y, H->closure_conveyance};
-            const int cb = std::min((H->closure_n + 63) / 64, 4096);
-            closure_kernel<<<cb, 256, 0, H->stream>>>(C, A.s.d, A.s.h);
-            HIP_TRY(hipGetLastError());
+                          H->cl_params, H->sx, H->sy, H->corr_x, H->corr_y, H->closure_conveyance, A.members,
+                          A.tile_member};
+            if (A.members || A.want_wave) {
+                const int cb = std::min((H->closure_n + 63) / 64, 4096);
+                closure_kernel<<<cb, 256, 0, H->stream>>>(C, A.s.d, A.s.h);
+                HIP_TRY(hipGetLastError());
+            }
             faces_kernel<float, FAST, 512, true><<<blocks, 512, 0, H->stream>>>(A);
             HIP_TRY(hipGetLastError());
             return 0;
