@@ -14,7 +14,7 @@ from wadi2d_amd.engine.ensemble import Ensemble, run_batched
 
 
 def coarse_run(coarse_scens: list, faces: list[CoarseFaces], closure: ClosureModel | None, backend: str = "hip",
-               log=None):  # fmt: skip
+               log=None, graph_steps: int = 64):  # fmt: skip
     """All coarse members in one device-resident run, with the closure on their crest faces or without it."""
     ens = Ensemble.from_scenarios(coarse_scens)
 
@@ -23,7 +23,7 @@ def coarse_run(coarse_scens: list, faces: list[CoarseFaces], closure: ClosureMod
         if log is not None:
             log(f"[closure] {n} faces corrected")
 
-    return run_batched(ens, backend, graph_steps=64, chunk=512, tune=False, launch=(512, 256), maxima_every=1,
+    return run_batched(ens, backend, graph_steps=graph_steps, chunk=512, tune=False, launch=(512, 256), maxima_every=1,
                        prepare=prepare if closure is not None else None, log=log), ens  # fmt: skip
 
 
