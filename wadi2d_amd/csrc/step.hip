@@ -1113,23 +1113,36 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.stepping = 1;
 }
 
-// Block-wide exclusive prefix of one flag per thread over a chunk of PRE_THREADS: returns this thread's slot and
-// adds the chunk's count to `base` (in every thread).
-__device__ __forceinline__ int chunk_slot(bool flag, int* wave_cnt, int& base) {
-    const unsigned long long mask = __ballot(flag);
+// Block-wide exclusive prefixes of three flags per thread over a chunk of PRE_THREADS, with one barrier pair:
+// returns this thread's slots and adds the chunk's counts to `base` (in every thread).
+__device__ __forceinline__ int3 chunk_slots(bool f0, bool f1, bool f2, int (*wave_cnt)[PRE_THREADS / 64], int3& base) {
+    const unsigned long long m0 = __ballot(f0), m1 = __ballot(f1), m2 = __ballot(f2);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int below = __popcll(mask & ((1ull << lane) - 1ull));
-    if (lane == 0) wave_cnt[wv] = __popcll(mask);
-    __syncthreads();
-    int off = 0, total = 0;
-    for (int q = 0; q < PRE_THREADS / 64; ++q) {
-        const int c = wave_cnt[q];
-        if (q < wv) off += c;
-        total += c;
+    const unsigned long long below = (1ull << lane) - 1ull;
+    if (lane == 0) {
+        wave_cnt[0][wv] = __popcll(m0);
+        wave_cnt[1][wv] = __popcll(m1);
+        wave_cnt[2][wv] = __popcll(m2);
     }
     __syncthreads();
-    const int slot = base + off + below;
-    base += total;
+    int3 off = make_int3(0, 0, 0), total = make_int3(0, 0, 0);
+    for (int q = 0; q < PRE_THREADS / 64; ++q) {
+        const int c0 = wave_cnt[0][q], c1 = wave_cnt[1][q], c2 = wave_cnt[2][q];
+        if (q < wv) {
+            off.x += c0;
+            off.y += c1;
+            off.z += c2;
+        }
+        total.x += c0;
+        total.y += c1;
+        total.z += c2;
+    }
+    __syncthreads();
+    const int3 slot = make_int3(base.x + off.x + __popcll(m0 & below), base.y + off.y + __popcll(m1 & below),
+                                base.z + off.z + __popcll(m2 & below));
+    base.x += total.x;
+    base.y += total.y;
+    base.z += total.z;
     return slot;
 }
 
@@ -1151,7 +1164,8 @@ struct RunLists {
 // their neighbours (retain); 2 does that and lets a tile go after SLEEP_CHECKS checks that did not want it (check);
 // 3 drops the member's tiles (done). The list stays in ascending order; woken and leaving tiles go to their lists.
 // One workgroup, the member's own tiles.
-__device__ void rebuild(Member& b, int mode, const RunLists& L, int* wave_cnt, int& n_woken, int& n_leaving) {
+__device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[PRE_THREADS / 64], int& n_woken,
+                        int& n_leaving) {
     const int t0 = b.tile0, t1 = b.tile1;
     const bool look = mode == 1 || mode == 2;
     for (int t = t0 + threadIdx.x; t < t1; t += PRE_THREADS) {
@@ -1182,7 +1196,7 @@ __device__ void rebuild(Member& b, int mode, const RunLists& L, int* wave_cnt, i
         L.flags[t] |= a << 1;
     }
     __syncthreads();
-    int na = 0, nw = 0, nl = 0;
+    int3 n = make_int3(0, 0, 0);  // active, woken, leaving
     for (int base = t0; base < t1; base += PRE_THREADS) {
         const int t = base + threadIdx.x;
         const bool in = t < t1;
@@ -1190,18 +1204,16 @@ __device__ void rebuild(Member& b, int mode, const RunLists& L, int* wave_cnt, i
         const int cur = in ? L.is_active[t] : 0;
         const bool wk = a && !cur && look;
         const bool lv = cur && !a;
-        const int sa = chunk_slot(a, wave_cnt, na);
-        const int sw = chunk_slot(wk, wave_cnt, nw);
-        const int sl = chunk_slot(lv, wave_cnt, nl);
-        if (a) L.seg[t0 + sa] = t;
-        if (wk) L.woken[t0 + sw] = t;
-        if (lv) L.leaving[t0 + sl] = t;
+        const int3 slot = chunk_slots(a, wk, lv, wave_cnt, n);
+        if (a) L.seg[t0 + slot.x] = t;
+        if (wk) L.woken[t0 + slot.y] = t;
+        if (lv) L.leaving[t0 + slot.z] = t;
         if (in) L.is_active[t] = a;
     }
     __syncthreads();
-    if (threadIdx.x == 0) b.nlist = na;
-    n_woken = nw;
-    n_leaving = nl;
+    if (threadIdx.x == 0) b.nlist = n.x;
+    n_woken = n.y;
+    n_leaving = n.z;
     __syncthreads();
 }
 
@@ -1238,7 +1250,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
                                                           const float2* __restrict__ ctab, size_t ncell, int last,
                                                           int flush_only) {
     static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
-    __shared__ int wave_cnt[PRE_THREADS / 64];
+    __shared__ int wave_cnt[3][PRE_THREADS / 64];
     __shared__ int any;
     Member& b = mem[blockIdx.x];
     int nw, nl;
@@ -1286,8 +1298,10 @@ __global__ __launch_bounds__(PRE_THREADS) void gather_kernel(const Member* __res
 // After a step, one workgroup per member: its boundary cells (cells.cu:95-111) and FP64 tally
 // (engine/maxima.py:53-62), the wave-speed reduction and the next step length with the failure rules
 // (stepper.py:109-126), the window's rain volume, the clock, and the check every active_every steps.
+constexpr int POST_THREADS = 1024;  // a member's tiles, boundary cells and tally faces are strided over them
+
 template <class R>
-__global__ __launch_bounds__(256) void post_kernel(Member* __restrict__ mem, const R* __restrict__ wave,
+__global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__ mem, const R* __restrict__ wave,
                                                    const int* __restrict__ is_active,
                                                    const double* __restrict__ src_partial, const RunNum num,
                                                    const int* __restrict__ cell, const int* __restrict__ ref,
@@ -1297,7 +1311,7 @@ __global__ __launch_bounds__(256) void post_kernel(Member* __restrict__ mem, con
                                                    const float* __restrict__ sign, const R* __restrict__ qx,
                                                    const R* __restrict__ qy, double* __restrict__ total,
                                                    double* __restrict__ flow) {
-    __shared__ double red_w[4], red_s[4];
+    __shared__ double red_w[POST_THREADS / 64], red_s[POST_THREADS / 64];
     Member& b = mem[blockIdx.x];
     if (!b.stepping) return;
     for (int p = b.b0 + threadIdx.x; p < b.b1; p += blockDim.x) {
@@ -1343,7 +1357,7 @@ __global__ __launch_bounds__(256) void post_kernel(Member* __restrict__ mem, con
     if (threadIdx.x != 0) return;
     w = red_w[0];
     rs = red_s[0];
-    for (int q = 1; q < 4; ++q) {
+    for (int q = 1; q < POST_THREADS / 64; ++q) {
         w = red_w[q] > w ? red_w[q] : w;
         rs += red_s[q];
     }
@@ -2366,7 +2380,7 @@ int enqueue_run_step(w2d_handle* H) {
     if (H->timing && (s = mark(H))) return s;
     if ((s = launch_cells<R, FAST>(H, A, nt))) return s;
     if (H->timing && (s = mark(H))) return s;
-    post_kernel<R><<<H->run_m, 256, 0, H->stream>>>(H->run_mem, (const R*)H->wave, H->run_is_active,
+    post_kernel<R><<<H->run_m, POST_THREADS, 0, H->stream>>>(H->run_mem, (const R*)H->wave, H->run_is_active,
                                                      H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
                                                      H->b_d0, H->b_h0, h_new, (R*)H->d, H->t_face, H->t_axis,
                                                      H->t_sign, (R*)H->qx, (R*)H->qy, H->t_total, H->t_flow);
