@@ -207,28 +207,6 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
     W = dv<FAST>(a1 - a0, de);
 }
 
-// Level and, when asked, wet share of a cell from one row pair read through the cell's row cache; the same
-// statements as cell_level and cell_plan below.
-template <class R, bool FAST>
-__device__ __forceinline__ R cell_eval(R hb, R Hc, const float2* __restrict__ tab, size_t plane, int last,
-                                       float2* cache, unsigned char* tag, bool want_plan, R& plan) {
-    plan = R(1);
-    if (Hc <= R(0)) return hb;
-    if (hb >= Hc) {
-        const int have = *tag;
-        return R(fetch_row(tab, plane, last - 1, cache, tag, have).y) + (hb - Hc);
-    }
-    const int have = *tag;
-    const R s = sq<FAST>(dv<FAST>(hb, Hc)) * R(last);
-    const int r = row_of(s, last - 1);
-    const float2 q = fetch_row(tab, plane, r, cache, tag, have);
-    if (want_plan) {
-        const R m = R(last);
-        plan = dv<FAST>(dv<FAST>(Hc * R(2 * r + 1), m * m), pmax(R(q.y) - R(q.x), R(1e-12)));
-    }
-    return R(q.x) + (s - R(r)) * (R(q.y) - R(q.x));
-}
-
 // Level above zmin from the mean depth, and the wet share of the plan (common.cuh:39-56).
 template <class R, bool FAST>
 __device__ __forceinline__ R cell_level(R hb, R Hc, const float2* __restrict__ tab, size_t plane, int last) {
@@ -856,37 +834,99 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     const bool want_plan = want_run != 0;
     int wet = 0;
     double src_sum = 0.0;
-#pragma unroll 1
-    for (int r = 0; r < TILE2 / NTH; ++r) {
+    // The thread's RT cells in three passes, so that the loads of one cell need not wait for the stores of the
+    // one before: (1) the cell's inputs, (2) continuity and the level table's row pair, (3) the stores.
+    constexpr int RT = TILE2 / NTH;
+    unsigned cs[RT];
+    R res_v[RT], src_v[RT];
+    float hc_v[RT];
+    int tag_v[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+        cs[r] = c;
+        res_v[r] = A.s.res[c];
+        hc_v[r] = A.st.hc[c];
+        tag_v[r] = A.st.ctag[c];
+        if (new_window)  // the window's rate of this cell: float(weight * scale), the product in double
+            src_v[r] = A.src_weight ? (R)(A.src_weight[c] * scale) : R(0);
+        else
+            src_v[r] = A.s.src[c];
+    }
+    R qw_v[RT], qn_v[RT], hn_v[RT], resn_v[RT], s_v[RT];
+    int row_v[RT], mode_v[RT];
+    float2 pair_v[RT];
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
-        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+        const unsigned c = cs[r];
         const R rc = s_r[L];
         const R qw = scaled(s_qx[L], s_r[L - 1], rc);
         const R qe = scaled(s_qx[L + 1], rc, s_r[L + 1]);
         const R qn = scaled(s_qy[L], s_r[L - LS], rc);
         const R qs = scaled(s_qy[L + LS], rc, s_r[L + LS]);
-        A.s.qx[c] = qw;
-        A.s.qy[c] = qn;
+        qw_v[r] = qw;
+        qn_v[r] = qn;
         const R h0 = s_h[L];
-        R src_c;
-        if (new_window) {
-            // the window's rate of this cell: float(weight * scale), the product in double, as the host forms it
-            src_c = A.src_weight ? (R)(A.src_weight[c] * scale) : R(0);
-            A.s.src[c] = src_c;
-            src_sum += (double)src_c;
-        } else {
-            src_c = A.s.src[c];
-        }
-        const R inc = A.s.res[c] + dt * src_c - dv<FAST>(dt * (qe - qw + qs - qn), area);
+        const R inc = res_v[r] + dt * src_v[r] - dv<FAST>(dt * (qe - qw + qs - qn), area);
         const R sum = h0 + inc;
         const R hn = pmax(sum, R(0));
-        A.s.res[c] = sum > R(0) ? inc - (sum - h0) : R(0);
+        resn_v[r] = sum > R(0) ? inc - (sum - h0) : R(0);
+        hn_v[r] = hn;
+        const R Hc = R(hc_v[r]);
+        // the level table's row for hn (the statements of cell_level and cell_plan below, the row pair read
+        // through the cell's row cache): 0 no table, 1 above its top, 2 within it
+        int mode = 0, row = 0;
+        R sv = R(0);
+        if (Hc > R(0)) {
+            if (hn >= Hc) {
+                mode = 1;
+                row = A.st.last - 1;
+            } else {
+                mode = 2;
+                sv = sq<FAST>(dv<FAST>(hn, Hc)) * R(A.st.last);
+                row = row_of(sv, A.st.last - 1);
+            }
+        }
+        mode_v[r] = mode;
+        row_v[r] = row;
+        s_v[r] = sv;
+        if (mode) pair_v[r] = tag_v[r] == row ? A.st.cc[c] : A.st.ctab[(size_t)row * A.st.ncell + c];
+    }
+#pragma unroll
+    for (int r = 0; r < RT; ++r) {
+        const unsigned c = cs[r];
+        A.s.qx[c] = qw_v[r];
+        A.s.qy[c] = qn_v[r];
+        if (new_window) {
+            A.s.src[c] = src_v[r];
+            src_sum += (double)src_v[r];
+        }
+        A.s.res[c] = resn_v[r];
+        const R hn = hn_v[r];
         A.s.h_next[c] = hn;
-        const R Hc = R(A.st.hc[c]);
-        R plan;
-        A.s.d[c] = cell_eval<R, FAST>(hn, Hc, A.st.ctab + c, A.st.ncell, A.st.last, A.st.cc + c, A.st.ctag + c,
-                                      want_plan, plan);
+        const R Hc = R(hc_v[r]);
+        const float2 q = pair_v[r];
+        R dn, plan = R(1);
+        if (mode_v[r] == 0) {
+            dn = hn;
+        } else if (mode_v[r] == 1) {
+            dn = R(q.y) + (hn - Hc);
+        } else {
+            const int row = row_v[r];
+            if (want_plan) {
+                const R m = R(A.st.last);
+                plan = dv<FAST>(dv<FAST>(Hc * R(2 * row + 1), m * m), pmax(R(q.y) - R(q.x), R(1e-12)));
+            }
+            dn = R(q.x) + (s_v[r] - R(row)) * (R(q.y) - R(q.x));
+        }
+        if (mode_v[r] && tag_v[r] != row_v[r]) {  // refill the cell's row cache
+            A.st.cc[c] = q;
+            A.st.ctag[c] = (unsigned char)row_v[r];
+        }
+        A.s.d[c] = dn;
         if (want_plan) A.s.om[c] = plan;
         wet |= hn > R(0);
     }
