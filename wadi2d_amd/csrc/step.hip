@@ -141,14 +141,14 @@ struct Params {
     int rows, second, sheet, mirror, carry;
 };
 
-// The row pair `r` of a table record, through the face's (or cell's) row cache when it has one: the cache holds
-// the pair of the row last read and its row in `tag`, so a lookup whose row has not changed since the last step
-// reads 17 coalesced bytes instead of gathering a 128-byte line; a row change reads the table and refills it.
+// The row pair `r` of a table record, through the face's row cache when it has one: the cache holds the pair of
+// the row last read and its row in `tag`; both were read (`pre`, `have`) before the row was known, so a lookup whose
+// row has not changed since the last step waits for no further load; a row change reads the table and refills it.
 template <class V>
 __device__ __forceinline__ V fetch_row(const V* __restrict__ tab, size_t plane, int r, V* cache, unsigned char* tag,
-                                       int have) {
+                                       int have, V pre) {
     if (cache) {
-        if (have == r) return *cache;
+        if (have == r) return pre;
         const V q = tab[(size_t)r * plane];
         *cache = q;
         *tag = (unsigned char)r;
@@ -159,14 +159,14 @@ __device__ __forceinline__ V fetch_row(const V* __restrict__ tab, size_t plane, 
 
 // Area, conveyance^(3/5), wet width at depth e over the invert (common.cuh:64-90): rows quadratic in e, read
 // linearly between them, prismatic above the table top E. `tab` points at this face in plane 0; planes are
-// `plane` records apart. `cache`/`tag`/`have`: the face's row cache (null for none), `have` its tag as read.
+// `plane` records apart. `cache`/`tag`/`have`/`pre`: the face's row cache (null for none), its tag and pair as read.
 template <class R, bool FAST>
 __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__ tab, size_t plane, int last, R& A,
                                             R& T, R& W, float4* cache = nullptr, unsigned char* tag = nullptr,
-                                            int have = -1) {
+                                            int have = -1, float4 pre = float4{0.f, 0.f, 0.f, 0.f}) {
     const R m = R(last);
     if (e >= E) {
-        const float4 q = fetch_row(tab, plane, last - 1, cache, tag, have);
+        const float4 q = fetch_row(tab, plane, last - 1, cache, tag, have, pre);
         const R a1 = R(q.z), a0 = R(q.x);
         const R c = dv<FAST>(m - R(1), m);
         W = dv<FAST>(a1 - a0, E * (R(1) - c * c));
@@ -179,7 +179,7 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
         const R x = e * __builtin_amdgcn_rcpf(E);
         const R s = sq<FAST>(x) * m;
         const int r = row_of(s, last - 1);
-        const float4 q = fetch_row(tab, plane, r, cache, tag, have);
+        const float4 q = fetch_row(tab, plane, r, cache, tag, have, pre);
         const R im = __builtin_amdgcn_rcpf(m);
         const R c0 = R(r) * im;
         const R c1 = R(r + 1) * im;
@@ -195,7 +195,7 @@ __device__ __forceinline__ void face_lookup(R e, R E, const float4* __restrict__
     }
     const R s = sq<FAST>(dv<FAST>(e, E)) * m;
     const int r = row_of(s, last - 1);
-    const float4 q = fetch_row(tab, plane, r, cache, tag, have);
+    const float4 q = fetch_row(tab, plane, r, cache, tag, have, pre);
     const R c0 = dv<FAST>(R(r), m);
     const R c1 = dv<FAST>(R(r + 1), m);
     const R e0 = c0 * c0 * E;
@@ -434,6 +434,9 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R eL = in.sd[L - S] - fl;
     const R eR = in.sd[L] - fr;
     const R uf = u[L];
+    // the face's row cache, read with the statics: the row is known only after the carry
+    const int have = cache ? (int)*tag : -1;
+    const float4 pre = cache ? *cache : float4{0.f, 0.f, 0.f, 0.f};
     const R eLp = fmx<FAST>(eL, R(0));
     const R eRp = fmx<FAST>(eR, R(0));
     const bool from_left = uf > R(0) || (uf == R(0) && eL >= eR);
@@ -444,7 +447,6 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
         q_out = R(0);
         return R(0);
     }
-    const int have = cache ? (int)*tag : -1;  // issued early: the row is known only after the carry
     R eface = eup;
     if (P.carry && cw == R(0)) {
         const R dz = from_left ? R(zc[k_up]) - fl : R(zc[k_own]) - fr;
@@ -455,7 +457,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
         }
     }
     R A, T, W;
-    face_lookup<R, FAST>(eface, ef, ftab, plane, last, A, T, W, cache, tag, have);
+    face_lookup<R, FAST>(eface, ef, ftab, plane, last, A, T, W, cache, tag, have, pre);
     if constexpr (LEARNED) T *= R(corr.x);
     const R hL = in.sh[L - S];
     const R hR = in.sh[L];
