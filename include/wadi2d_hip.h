@@ -109,7 +109,6 @@ int w2d_source_sum(w2d_handle* h, double* out);
  * face and cell kernels; timing_read waits, and returns and clears the summed milliseconds and the step count. */
 int w2d_timing(w2d_handle* h, int32_t on);
 int w2d_timing_read(w2d_handle* h, double* faces_ms, double* cells_ms, int64_t* launches);
-int w2d_timing_read_marks(w2d_handle* h, double* faces_ms, double* cells_ms, int64_t* launches);
 /* A plain device copy of `bytes` (no counterpart): the yardstick that scales the memory counters; returns the mean
  * milliseconds per copy over `repeats` copies. */
 int w2d_copy_probe(int64_t bytes, int32_t repeats, double* ms);

@@ -70,7 +70,10 @@ class Buffer:
     def __del__(self) -> None:
         p = getattr(self, "ptr", None)
         if p is not None and p.value:
-            api().w2d_dev_free(p)
+            try:
+                api().w2d_dev_free(p)
+            except Exception:  # interpreter shutdown: the library may already be gone
+                pass
             self.ptr = ctypes.c_void_p()
 
 

@@ -12,6 +12,8 @@
 
 using namespace w2d_mlp;
 
+int w2d_report(int code, const char* msg);  // csrc/step.hip: the message w2d_last_error returns
+
 namespace {
 
 constexpr int MT = 256;             // threads per workgroup: 4 waves
@@ -216,41 +218,49 @@ int blocks_for(long long n, int max_blocks) {
     return b < 1 ? 1 : (int)b;
 }
 
-int check_launch() { return hipGetLastError() == hipSuccess ? 0 : 3; }
+int check_launch() {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? 0 : w2d_report(3, hipGetErrorString(e));
+}
 
 }  // namespace
 
 extern "C" {
 
 int w2d_dev_alloc(int64_t bytes, void** out) {
-    if (!out || bytes < 0) return 1;
+    if (!out || bytes < 0) return w2d_report(1, "dev_alloc: bad arguments");
     *out = nullptr;
     if (bytes == 0) return 0;
     if (hipMalloc(out, (size_t)bytes) != hipSuccess) {
         (void)hipGetLastError();
         *out = nullptr;
-        return 2;
+        return w2d_report(2, "dev_alloc: out of device memory");
     }
     return 0;
 }
 
 int w2d_dev_free(void* p) {
-    if (p && hipFree(p) != hipSuccess) return 3;
+    if (p && hipFree(p) != hipSuccess) return w2d_report(3, "dev_free: hipFree failed");
     return 0;
 }
 
 int w2d_dev_copy(void* dst, const void* src, int64_t bytes, int32_t kind) {
-    if ((!dst || !src) && bytes) return 1;
-    if (bytes <= 0) return bytes < 0 ? 1 : 0;
+    if (((!dst || !src) && bytes) || bytes < 0 || kind < 0 || kind > 2) return w2d_report(1, "dev_copy: bad arguments");
+    if (bytes == 0) return 0;
     const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice;
-    return hipMemcpy(dst, src, (size_t)bytes, k) == hipSuccess ? 0 : 3;
+    const hipError_t e = hipMemcpy(dst, src, (size_t)bytes, k);
+    return e == hipSuccess ? 0 : w2d_report(3, hipGetErrorString(e));
 }
 
-int w2d_dev_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : 3; }
+int w2d_dev_synchronize(void) {
+    const hipError_t e = hipDeviceSynchronize();
+    return e == hipSuccess ? 0 : w2d_report(3, hipGetErrorString(e));
+}
 
 int w2d_mlp_take(const uint16_t* x, const float* target, const float* mask, const int64_t* idx, int64_t n,
                  uint16_t* x_out, float* target_out, float* mask_out, void* stream) {
-    if (n < 0 || (n && (!x || !target || !mask || !idx || !x_out || !target_out || !mask_out))) return 1;
+    if (n < 0 || (n && (!x || !target || !mask || !idx || !x_out || !target_out || !mask_out)))
+        return w2d_report(1, "mlp_take: bad arguments");
     if (n == 0) return 0;
     take_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(x, target, mask, idx, n, x_out,
                                                                              target_out, mask_out);
@@ -258,28 +268,28 @@ int w2d_mlp_take(const uint16_t* x, const float* target, const float* mask, cons
 }
 
 int w2d_mlp_sizes(int32_t* nparam, int32_t* nbf16) {
-    if (!nparam || !nbf16) return 1;
+    if (!nparam || !nbf16) return w2d_report(1, "mlp_sizes: bad arguments");
     *nparam = NPARAM;
     *nbf16 = NBF16;
     return 0;
 }
 
 int w2d_mlp_pack(const float* params, uint16_t* wbf16, void* stream) {
-    if (!params || !wbf16) return 1;
+    if (!params || !wbf16) return w2d_report(1, "mlp_pack: bad arguments");
     pack_kernel<<<(HID * HID + 255) / 256, 256, 0, (hipStream_t)stream>>>(params, wbf16);
     return check_launch();
 }
 
 int w2d_mlp_forward(const uint16_t* wbf16, const float* params, const uint16_t* x, int64_t n, float* y,
                     void* stream) {
-    if (!wbf16 || !params || (n && (!x || !y)) || n < 0) return 1;
+    if (!wbf16 || !params || (n && (!x || !y)) || n < 0) return w2d_report(1, "mlp_forward: bad arguments");
     if (n == 0) return 0;
     forward_kernel<<<blocks_for(n, 4096), MT, 0, (hipStream_t)stream>>>(wbf16, params, x, n, y);
     return check_launch();
 }
 
 int w2d_mlp_train_blocks(int64_t n, int32_t* blocks) {
-    if (!blocks || n < 0) return 1;
+    if (!blocks || n < 0) return w2d_report(1, "mlp_train_blocks: bad arguments");
     *blocks = blocks_for(n, 512);
     return 0;
 }
@@ -288,8 +298,8 @@ int w2d_mlp_gradient(const uint16_t* wbf16, const float* params, const uint16_t*
                      const float* mask, int64_t n, float norm, float* partial, int32_t blocks, float* grad,
                      void* stream) {
     if (!wbf16 || !params || !x || !target || !mask || !partial || !grad || n < 1 || blocks < 1 || !(norm > 0.f))
-        return 1;
-    if (blocks != blocks_for(n, 512)) return 1;
+        return w2d_report(1, "mlp_gradient: bad arguments");
+    if (blocks != blocks_for(n, 512)) return w2d_report(1, "mlp_gradient: blocks is not w2d_mlp_train_blocks(n)");
     hipStream_t st = (hipStream_t)stream;
     train_kernel<<<blocks, MT, 0, st>>>(wbf16, params, x, target, mask, n, norm, partial);
     if (int s = check_launch()) return s;
@@ -299,7 +309,7 @@ int w2d_mlp_gradient(const uint16_t* wbf16, const float* params, const uint16_t*
 
 int w2d_mlp_adam(float* params, float* m, float* v, const float* grad, int32_t step, float lr, float beta1,
                  float beta2, float eps, uint16_t* wbf16, void* stream) {
-    if (!params || !m || !v || !grad || !wbf16 || step < 1) return 1;
+    if (!params || !m || !v || !grad || !wbf16 || step < 1) return w2d_report(1, "mlp_adam: bad arguments");
     const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
     hipStream_t st = (hipStream_t)stream;
     adam_kernel<<<(NPARAM + 255) / 256, 256, 0, st>>>(params, m, v, grad, lr, beta1, beta2, eps, c1, c2, wbf16);

@@ -1771,6 +1771,10 @@ int big_sum(w2d_handle* H, int64_t n, const int64_t* index, const R* a, const R*
 
 }  // namespace
 
+// Shared with the library's other translation unit (csrc/mlp.hip): record the message w2d_last_error returns, and
+// return `code`.
+int w2d_report(int code, const char* msg) { return fail(code, msg); }
+
 extern "C" {
 
 const char* w2d_last_error(void) { return g_error.c_str(); }
@@ -2249,14 +2253,15 @@ int w2d_timing(w2d_handle* H, int32_t on) {
     return 0;
 }
 
-int w2d_timing_read_marks(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches);
+static int timing_collect(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches);
 
 int w2d_timing_read(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches) {
     if (!H || !faces_ms || !cells_ms || !launches) return fail(1, "timing_read: bad arguments");
-    return w2d_timing_read_marks(H, faces_ms, cells_ms, launches);
+    return timing_collect(H, faces_ms, cells_ms, launches);
 }
 
-int w2d_timing_read_marks(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches) {
+// The summed milliseconds between the recorded marks (three per step), cleared; the caller checked the arguments.
+static int timing_collect(w2d_handle* H, double* faces_ms, double* cells_ms, int64_t* launches) {
     HIP_TRY(hipStreamSynchronize(H->stream));
     double f = 0.0, c = 0.0;
     const size_t n = H->marks.size() / 3;
