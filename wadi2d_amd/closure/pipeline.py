@@ -28,7 +28,7 @@ def judge(model: ClosureModel | None, col, pairs) -> dict:
     return errors(run.max_depth, col.dmax, col.coarse).__dict__
 
 
-CALIBRATION = [(conv, scale) for conv in (False, True) for scale in (0.25, 0.5, 0.75, 1.0)]
+CALIBRATION = [(conv, scale) for conv in (False, True) for scale in (0.1, 0.2, 0.3, 0.4, 0.5, 0.75, 1.0)]
 
 
 def calibrate(model: ClosureModel, col, pairs) -> tuple[ClosureModel, dict]:
