@@ -16,7 +16,7 @@ is the oracle's float instantiation (the reference's float32 statement order, Op
 cores, timed on a bounded number of steps from the same hot state.
 
 Other workloads (``--workload``): ``ensemble``, member-hours per wall-hour of 64 uneven storm/ARF members run
-together on the device against the same members run one after another the plain way, both judged against the
+together on the device against the same members run one after another the plain way, both held against the
 members run together in FP64; ``precision``, the steep 2048 x 2048 reservoir break in FP32 against FP64;
 ``learned``, the learned closure inside the batched launch against the plain table path.
 """
@@ -67,7 +67,7 @@ def main() -> int:
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
     ap.add_argument("--serial-members", type=int, default=-1, help="members the serial baseline runs (-1: all)")
     ap.add_argument("--reference", choices=("on", "off"), default="on", help="the FP64 batched run (ensemble)")
-    ap.add_argument("--judge-members", type=int, default=8, help="learned: held-out members with 2 m runs")
+    ap.add_argument("--check-members", type=int, default=8, help="learned: held-out members with 2 m runs")
     ap.add_argument("--held-out", type=int, default=32, help="learned: held-out 1024x1024 members in the launch")
     ap.add_argument("--workers", type=int, default=8, help="learned: processes building the members' tables")
     ap.add_argument("--maxima-every", type=int, default=4,
@@ -215,7 +215,7 @@ def main() -> int:
 def ensemble(args) -> int:
     """Member-hours per wall-hour of the storm/ARF ensemble: all members in one device-resident run (launch shape
     from the tuner) against the plain way, each member alone with its step loop on the host, one after another.
-    Both in fast FP32; both judged by max depth against the same members run together in FP64."""
+    Both in fast FP32; both held by max depth against the same members run together in FP64."""
     from wadi2d_amd.engine.ensemble import Ensemble, draw_members, max_depth_error, run_batched, run_serial
 
     t0 = time.time()
@@ -375,16 +375,16 @@ def learned(args) -> int:
     rate = hours / (got.wall_s / 3600.0)
     rate_plain_batched = hours / (plain.wall_s / 3600.0)
     rate_serial = 3.0 * n_serial / (serial.wall_s / 3600.0) if n_serial else None
-    judge = {}
-    if args.judge_members > 0:
+    check = {}
+    if args.check_members > 0:
         from wadi2d_amd.closure.data import middle_runs
 
-        col, pairs = fine_runs(list(range(5000, 5000 + args.judge_members)), 64, 5400.0, chunk=64, storm_seed=1)
+        col, pairs = fine_runs(list(range(5000, 5000 + args.check_members)), 64, 5400.0, chunk=64, storm_seed=1)
         cs = [c for _, c in pairs]
         jp, _ = coarse_run(cs, col.coarse, None)
         jl, _ = coarse_run(cs, col.coarse, model)
         mid, mrun = middle_runs(pairs, col.coarse, 64, 2, 5400.0)
-        judge = {"plain_20m": errors(jp.max_depth, col.dmax, col.coarse).__dict__,
+        check = {"plain_20m": errors(jp.max_depth, col.dmax, col.coarse).__dict__,
                  "learned_20m": errors(jl.max_depth, col.dmax, col.coarse).__dict__,
                  "plain_10m": errors(mid.dmax, col.dmax, col.coarse).__dict__,
                  "wall_s": {"plain_20m": jp.wall_s, "learned_20m": jl.wall_s, "plain_10m": mrun.wall_s}}  # fmt: skip
@@ -411,7 +411,7 @@ def learned(args) -> int:
             "speedup_vs_serial_plain": rate / rate_serial if rate_serial else None,
             "closure_cost": got.wall_s / plain.wall_s - 1.0,
             "max_depth_change_learned_vs_plain_m": diff,
-            "judge_against_2m_runs": judge,
+            "against_2m_runs": check,
             "wall_s": {"plain_batched": plain.wall_s, "learned_batched": got.wall_s, "serial": serial_wall},
         },
         "roofline": None,

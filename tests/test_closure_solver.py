@@ -102,7 +102,7 @@ def test_a_closure_that_raises_a_line_of_faces_holds_the_water():
 @pytest.mark.gpu
 def test_the_trained_closure_beats_the_plain_table_path_on_unseen_ground():
     """The committed network on four members it never saw (seeds apart from training and calibration): their 20 m
-    runs with and without it against their own 2 m runs coarse-grained. The plain table path is the judge: the
+    runs with and without it against their own 2 m runs coarse-grained. The plain table path is the yardstick: the
     learned run must cut its max-depth RMSE by at least 15 %."""
     from wadi2d_amd.closure.data import fine_runs
     from wadi2d_amd.closure.evaluate import coarse_run, errors

@@ -1,4 +1,4 @@
-"""The closure judged: coarse runs of held-out members with and without it against the same members' fine-grid
+"""The closure checked: coarse runs of held-out members with and without it against the same members' fine-grid
 runs coarse-grained (closure/data.py), and the plain table path as the reference point. Errors are over the coarse
 cells the water reaches in either run (max level over 2 cm), the free outflow column left out."""
 

@@ -1,5 +1,5 @@
 """Build the closure end to end on one GPU: fine-grid runs of training members, coarse-grained samples, training on
-the GPU, and the judgement on held-out members against their own fine-grid runs, with and without the closure.
+the GPU, and the check on held-out members against their own fine-grid runs, with and without the closure.
 
     python -m wadi2d_amd.closure.pipeline --out wadi2d_amd/closure/trained.npz
 """
@@ -22,7 +22,7 @@ def log(*a) -> None:
     print(*a, file=sys.stderr, flush=True)
 
 
-def judge(model: ClosureModel | None, col, pairs) -> dict:
+def score(model: ClosureModel | None, col, pairs) -> dict:
     """Errors of a coarse run (plain when ``model`` is None) against the coarse-grained fine runs in ``col``."""
     run, _ = coarse_run([c for _, c in pairs], col.coarse, model, log=None)
     return errors(run.max_depth, col.dmax, col.coarse).__dict__
@@ -33,11 +33,11 @@ CALIBRATION = [(conv, scale) for conv in (False, True) for scale in (0.1, 0.2, 0
 
 def calibrate(model: ClosureModel, col, pairs) -> tuple[ClosureModel, dict]:
     """The conveyance switch and the rise scale that give the least max-depth RMSE on the training members."""
-    scores = {"plain": judge(None, col, pairs)}
+    scores = {"plain": score(None, col, pairs)}
     best, best_rmse = None, float("inf")
     for conv, scale in CALIBRATION:
         m = model.with_(conveyance=conv, rise_scale=scale)
-        e = judge(m, col, pairs)
+        e = score(m, col, pairs)
         scores[f"conveyance={conv},rise={scale}"] = e
         log(f"[calibrate] conveyance {conv} rise x{scale}: rmse {e['rmse']:.4f} (plain {scores['plain']['rmse']:.4f})")
         if e["rmse"] < best_rmse:
@@ -48,7 +48,7 @@ def calibrate(model: ClosureModel, col, pairs) -> tuple[ClosureModel, dict]:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--train-members", type=int, default=32)
-    ap.add_argument("--judge-members", type=int, default=8)
+    ap.add_argument("--held-out-members", type=int, default=8)
     ap.add_argument("--n", type=int, default=64, help="coarse cells per side of a member (20 m)")
     ap.add_argument("--duration", type=float, default=5400.0)
     ap.add_argument("--steps", type=int, default=4000)
@@ -69,10 +69,10 @@ def main() -> int:
     log(f"[pipeline] chosen: conveyance {model.conveyance}, rise x{model.rise_scale}")
     if args.out:
         model.save(args.out)
-    held = list(range(5000, 5000 + args.judge_members))
+    held = list(range(5000, 5000 + args.held_out_members))
     hcol, hpairs = fine_runs(held, args.n, args.duration, chunk=64, log=log, storm_seed=1)
-    verdict = {"plain": judge(None, hcol, hpairs), "learned": judge(model, hcol, hpairs)}
-    print(json.dumps({"samples": int(x.shape[0]), "history": history, "calibration": scores, "judge": verdict,
+    verdict = {"plain": score(None, hcol, hpairs), "learned": score(model, hcol, hpairs)}
+    print(json.dumps({"samples": int(x.shape[0]), "history": history, "calibration": scores, "held_out": verdict,
                       "chosen": {"conveyance": model.conveyance, "rise_scale": model.rise_scale},
                       "wall_s": time.time() - t0}), flush=True)  # fmt: skip
     return 0
