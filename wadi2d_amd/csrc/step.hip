@@ -136,6 +136,19 @@ __device__ __forceinline__ R ustar(R uc, R uup, R udown, bool second) {
 template <class R>
 __device__ __forceinline__ R gated(R n, R own, bool mirror) { return (mirror && n == R(0)) ? own : n; }
 
+// Streams a step touches once (per-cell statics, row caches, the kernels' outputs) go through nontemporal loads and
+// stores, so that they do not push out of L2 the arrays whose 2-cell rings neighbouring workgroups read again.
+// Measured on the 4096 x 4096 step: face kernel 0.534 -> 0.496 ms per launch.
+template <class T>
+__device__ __forceinline__ T ld_nt(const T* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float4 ld_nt(const float4* p) {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+template <class T>
+__device__ __forceinline__ void st_nt(T v, T* p) { __builtin_nontemporal_store(v, p); }
+
 struct Params {
     double g, dx, e_wet, h_cfl, u_max, lead, a_min;
     int rows, second, sheet, mirror, carry;
@@ -435,8 +448,8 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R eR = in.sd[L] - fr;
     const R uf = u[L];
     // the face's row cache, read with the statics: the row is known only after the carry
-    const int have = cache ? (int)*tag : -1;
-    const float4 pre = cache ? *cache : float4{0.f, 0.f, 0.f, 0.f};
+    const int have = cache ? (int)ld_nt(tag) : -1;
+    const float4 pre = cache ? ld_nt(cache) : float4{0.f, 0.f, 0.f, 0.f};
     const R eLp = fmx<FAST>(eL, R(0));
     const R eRp = fmx<FAST>(eR, R(0));
     const bool from_left = uf > R(0) || (uf == R(0) && eL >= eR);
@@ -667,30 +680,30 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     const bool want_wave = want_run != 0;
     const R dt = dt_run;
     R w = R(0);
-#pragma unroll 1
+#pragma unroll
     for (int r = 0; r < TILE2 / NTH; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
-        const float4 e = A.st.ec[c];
+        const float4 e = ld_nt(A.st.ec + c);
         R un, mn, qn;
         float2 cx{1.f, 0.f}, cy{1.f, 0.f};
         if constexpr (LEARNED) {
             cx = A.corr_x[c];
             cy = A.corr_y[c];
         }
-        const R wx = face<R, FAST, LEARNED>(fx, L, A.s.mx[c], A.st.sx[c], R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
+        const R wx = face<R, FAST, LEARNED>(fx, L, ld_nt(A.s.mx + c), ld_nt(A.st.sx + c), R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
                                             A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
                                             A.s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
-        A.s.u1[c] = un;
-        A.s.mx1[c] = mn;
-        A.s.qx1[c] = qn;
-        const R wy = face<R, FAST, LEARNED>(fy, L, A.s.my[c], A.st.sy[c], R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
+        st_nt(un, A.s.u1 + c);
+        st_nt(mn, A.s.mx1 + c);
+        st_nt(qn, A.s.qx1 + c);
+        const R wy = face<R, FAST, LEARNED>(fy, L, ld_nt(A.s.my + c), ld_nt(A.st.sy + c), R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
                                             A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
                                             A.s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
-        A.s.v1[c] = un;
-        A.s.my1[c] = mn;
-        A.s.qy1[c] = qn;
+        st_nt(un, A.s.v1 + c);
+        st_nt(mn, A.s.my1 + c);
+        st_nt(qn, A.s.qy1 + c);
         const R m = wx > wy ? wx : wy;
         w = m > w ? m : w;
     }
@@ -848,13 +861,13 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
         cs[r] = c;
-        res_v[r] = A.s.res[c];
-        hc_v[r] = A.st.hc[c];
-        tag_v[r] = A.st.ctag[c];
+        res_v[r] = ld_nt(A.s.res + c);
+        hc_v[r] = ld_nt(A.st.hc + c);
+        tag_v[r] = ld_nt(A.st.ctag + c);
         if (new_window)  // the window's rate of this cell: float(weight * scale), the product in double
             src_v[r] = A.src_weight ? (R)(A.src_weight[c] * scale) : R(0);
         else
-            src_v[r] = A.s.src[c];
+            src_v[r] = ld_nt(A.s.src + c);
     }
     R qw_v[RT], qn_v[RT], hn_v[RT], resn_v[RT], s_v[RT];
     int row_v[RT], mode_v[RT];
@@ -900,15 +913,15 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const unsigned c = cs[r];
-        A.s.qx[c] = qw_v[r];
-        A.s.qy[c] = qn_v[r];
+        st_nt(qw_v[r], A.s.qx + c);
+        st_nt(qn_v[r], A.s.qy + c);
         if (new_window) {
             A.s.src[c] = src_v[r];
             src_sum += (double)src_v[r];
         }
-        A.s.res[c] = resn_v[r];
+        st_nt(resn_v[r], A.s.res + c);
         const R hn = hn_v[r];
-        A.s.h_next[c] = hn;
+        st_nt(hn, A.s.h_next + c);
         const R Hc = R(hc_v[r]);
         const float2 q = pair_v[r];
         R dn, plan = R(1);
@@ -928,8 +941,8 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
             A.st.cc[c] = q;
             A.st.ctag[c] = (unsigned char)row_v[r];
         }
-        A.s.d[c] = dn;
-        if (want_plan) A.s.om[c] = plan;
+        st_nt(dn, A.s.d + c);
+        if (want_plan) st_nt(plan, A.s.om + c);
         wet |= hn > R(0);
     }
     if (wet) any_wet = 1;
