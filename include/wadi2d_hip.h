@@ -90,12 +90,18 @@ int w2d_tally_add(w2d_handle* h, double dt);
 int w2d_tally_volume(w2d_handle* h, double* out);
 /* Every face's own running total (n as set), for books per ensemble member. */
 int w2d_tally_totals(w2d_handle* h, double* out);
+/* Puts saved per-face totals back, n the tally's face count (engine/maxima.py:189-192 Tally.restore, for a
+ * checkpoint's resume, io/checkpoint.py:73). */
+int w2d_tally_restore(w2d_handle* h, const double* totals, int32_t n);
 /* The instantaneous flow through the tally's faces, sum of sign * q, m3/s (no counterpart: the outfall discharge). */
 int w2d_tally_flow(w2d_handle* h, double* out);
 /* Maxima (engine/maxima.py:28-51,103-146): depth, mean depth, speed, hazard, arrival; which = 0..4. */
 int w2d_maxima_set(w2d_handle* h, double arrival_depth, double speed_depth);
 int w2d_maxima_update(w2d_handle* h, double t);
 int w2d_maxima_get(w2d_handle* h, int32_t which, void* host, int64_t n);
+/* Puts saved maxima back, state layout as maxima_get gives them (engine/maxima.py:136-138: Maxima.restore, for a
+ * checkpoint's resume, io/checkpoint.py:59-75). */
+int w2d_maxima_put(w2d_handle* h, int32_t which, const void* host, int64_t n);
 /* Monitor.storage (engine/monitor.py:214-218) without the dx^2: sum of h + res over listed own cells, FP64. */
 int w2d_storage_set(w2d_handle* h, const int64_t* index, int64_t n);
 int w2d_storage(w2d_handle* h, double* out);

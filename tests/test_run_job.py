@@ -1,0 +1,372 @@
+"""``wadi2d run``: the job file, the DEM, the lines, the exit codes and progress events (wadi2d/run.py,
+wadi2d/engine/cli.py, tests/engine/test_run.py and test_resume_exact.py of the reference are the model).
+
+CPU tests cover what needs no device: the job schema (a reference-shaped job loads; what this path does not run is
+refused with exit 2), the DEM readers, the cells and faces under lines, the storm curves, and the exit codes of a run
+that cannot start (3 input missing, 4 no device). GPU tests run a small job end to end against the oracle driven
+through the same window loop (bit for bit in the exact float32 variant) and check that a stopped and resumed run
+equals the unbroken one bit for bit.
+"""
+
+from __future__ import annotations
+
+import json
+import subprocess
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from wadi2d_amd import job as jobmod
+from wadi2d_amd.dem import grid_over, read_raster
+from wadi2d_amd.engine.lines import cells_along, inside, line_faces
+from wadi2d_amd.grid import TileGrid
+
+ROOT = Path(__file__).resolve().parent.parent
+X0, Y0 = 1000.0, 2000.0  # south-west corner of the test DEM
+NY, NX, K, DX = 64, 96, 2, 10.0  # cells, pixels per cell, cell size
+
+
+def ground(seed: int = 3) -> np.ndarray:
+    """Pixels (rows north to south) of a plane falling 1 % east with roughness and a wall across x = 480 m with a
+    30 m gap: NY*K x NX*K pixels of DX/K m."""
+    rng = np.random.default_rng(seed)
+    p = DX / K
+    x = (np.arange(NX * K) + 0.5) * p
+    y = (np.arange(NY * K) + 0.5) * p
+    z = 100.0 - 0.01 * x[None, :] + 0.05 * rng.standard_normal((y.size, x.size))
+    wall = (x >= 480.0) & (x < 490.0)
+    gap = np.abs(y - 320.0) < 15.0
+    z[np.ix_(~gap, wall)] += 1.0
+    return z
+
+
+def write_asc(path: Path, z: np.ndarray, pixel: float, nodata: float | None = None, center: bool = False) -> None:
+    ny, nx = z.shape
+    with path.open("w", encoding="ascii") as f:
+        f.write(f"ncols {nx}\nnrows {ny}\n")
+        if center:
+            f.write(f"xllcenter {X0 + 0.5 * pixel}\nyllcenter {Y0 + 0.5 * pixel}\n")
+        else:
+            f.write(f"xllcorner {X0}\nyllcorner {Y0}\n")
+        f.write(f"cellsize {pixel}\n")
+        if nodata is not None:
+            f.write(f"NODATA_value {nodata}\n")
+        np.savetxt(f, z, fmt="%.4f")
+
+
+def job_dict(dem: str = "ground.asc", duration: float = 600.0) -> dict:
+    xe, yn = X0 + NX * DX, Y0 + NY * DX
+    return {
+        "project": {"name": "test", "description": ""},
+        "grid": {"dx": DX},
+        "terrain": {"dem": dem, "manning": 0.04},
+        "tables": {"rows": 8},
+        "numerics": {},
+        "storm": {"kind": "uniform", "depth_mm": 40.0, "duration_min": 20.0},
+        "boundaries": [{"kind": "free", "coordinates": [[xe - 1.0, yn], [xe - 1.0, Y0]], "name": "east"}],
+        "initial_levels": [{"coordinates": [[X0, yn], [X0 + 200.0, yn], [X0 + 200.0, yn - 200.0], [X0, yn - 200.0]],
+                            "level": 100.2}],
+        "outputs": {"maxima_every": 1, "series_every_s": 30.0, "map_every_s": 300.0, "progress_every_s": 0.5,
+                    "points": [{"name": "p1", "x": X0 + 300.0, "y": Y0 + 320.0}],
+                    "lines": [{"name": "gap", "coordinates": [[X0 + 470.0, Y0], [X0 + 470.0, yn]]}]},
+        "run": {"duration_s": duration, "source_every_s": 30.0, "checkpoint_every_s": 0.0},
+    }  # fmt: skip
+
+
+@pytest.fixture()
+def job_folder(tmp_path):
+    write_asc(tmp_path / "ground.asc", ground(), DX / K)
+    (tmp_path / "job.json").write_text(json.dumps(job_dict()), encoding="utf-8")
+    return tmp_path
+
+
+# ---- the job file
+
+
+def test_a_job_loads_with_its_dem_path_made_absolute(job_folder):
+    cfg = jobmod.load_job(job_folder / "job.json")
+    assert Path(cfg.terrain.dem) == (job_folder / "ground.asc").resolve()
+    assert cfg.tables.rows == 8 and cfg.numerics.cfl == 0.45 and cfg.run.source_every_s == 30.0
+
+
+def test_a_reference_shaped_job_with_inert_sections_loads(job_folder):
+    d = job_dict()
+    d["mesh"] = {"enabled": False, "coarse_dx": 40.0}
+    d["strips"] = {"enabled": False}
+    d["hydrology"] = {"subbasins": [], "hms_basin": None, "time_step_min": 5.0}
+    d["transmission"] = {"enabled": False, "rate_mm_h": 10.0}
+    d["outputs"]["live_every_s"] = 5.0
+    d["outputs"]["pixels"] = {"enabled": False}
+    d["numerics"] = {"local_time_stepping": False, "lts_levels": 3}
+    d["terrain"]["join_cells"] = 2.0
+    d["initial_points"], d["dam_levels"], d["breaches"] = [], [], []
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    cfg = jobmod.load_job(job_folder / "job.json")
+    said = jobmod.ignored(cfg)
+    assert "outputs.live_every_s" in said and "terrain.join_cells" in said and "hydrology" in said
+
+
+@pytest.mark.parametrize(
+    "edit, named",
+    [
+        (lambda d: d["terrain"].update(objects=[{"kind": "crest"}]), "terrain.objects"),
+        (lambda d: d.update(mesh={"enabled": True}), "mesh.enabled"),
+        (lambda d: d["boundaries"].append({"kind": "flow", "coordinates": [[0, 0], [1, 1]], "name": "in",
+                                           "flow": {"t": [0], "q": [1]}}), "flow"),  # fmt: skip
+        (lambda d: d["storm"].update(kind="gridded"), "gridded"),
+        (lambda d: d["numerics"].update(local_time_stepping=True), "numerics.local_time_stepping"),
+        (lambda d: d.update(breaches=[{"name": "b"}]), "breaches"),
+        (lambda d: d["grid"].update(domain="outline.gpkg"), "grid.domain"),
+    ],
+)
+def test_what_this_path_does_not_run_is_refused_as_an_invalid_job(job_folder, edit, named):
+    d = job_dict()
+    edit(d)
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    with pytest.raises(jobmod.InvalidJob) as info:
+        jobmod.load_job(job_folder / "job.json")
+    assert info.value.code == 2 and named in str(info.value)
+
+
+def test_a_bad_value_or_a_missing_file_has_its_exit_code(tmp_path):
+    with pytest.raises(jobmod.InputMissing) as info:
+        jobmod.load_job(tmp_path / "nothing.json")
+    assert info.value.code == 3
+    d = job_dict()
+    d["numerics"]["cfl"] = 5.0  # above the schema's bound
+    (tmp_path / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    with pytest.raises(jobmod.InvalidJob):
+        jobmod.load_job(tmp_path / "job.json")
+    (tmp_path / "job.json").write_text("{not json", encoding="utf-8")
+    with pytest.raises(jobmod.InvalidJob):
+        jobmod.load_job(tmp_path / "job.json")
+
+
+def test_the_job_hash_follows_the_water_and_not_the_outputs(job_folder):
+    a = jobmod.load_job(job_folder / "job.json")
+    b = a.model_copy(update={"outputs": a.outputs.model_copy(update={"map_every_s": 60.0})})
+    c = a.model_copy(update={"terrain": a.terrain.model_copy(update={"manning": 0.05})})
+    assert jobmod.job_hash(a) == jobmod.job_hash(b)
+    assert jobmod.job_hash(a) != jobmod.job_hash(c)
+
+
+# ---- the DEM
+
+
+def test_asc_and_npz_give_the_same_grid_and_no_data_leaves_cells_out(tmp_path):
+    z = ground()
+    write_asc(tmp_path / "a.asc", z, DX / K)
+    write_asc(tmp_path / "c.asc", z, DX / K, center=True)
+    np.savez(tmp_path / "b.npz", z=z, x0=X0, y0=Y0 + NY * DX, pixel=DX / K)
+    ra, rb, rc = read_raster(tmp_path / "a.asc"), read_raster(tmp_path / "b.npz"), read_raster(tmp_path / "c.asc")
+    for r in (rb, rc):
+        assert np.allclose(r.z, ra.z, atol=1e-4) and r.x0 == pytest.approx(ra.x0) and r.y0 == pytest.approx(ra.y0)
+    grid, bed = grid_over(ra, DX)
+    assert grid.mask.sum() == NY * NX
+    assert grid.x0 == pytest.approx(X0 - DX) and grid.y0 == pytest.approx(Y0 + NY * DX + DX)
+    # the bed's pixel (r, c) of the padded grid is raster pixel (r - K, c - K)
+    assert np.array_equal(bed.read(K, K, 3, 3), ra.z[:3, :3])
+    zn = z.copy()
+    zn[5, 7] = -9999.0  # cell (2, 3)
+    write_asc(tmp_path / "n.asc", zn, DX / K, nodata=-9999.0)
+    grid_n, _ = grid_over(read_raster(tmp_path / "n.asc"), DX)
+    assert grid_n.mask.sum() == NY * NX - 1 and not grid_n.mask[1 + 2, 1 + 3]
+
+
+def test_an_extent_snaps_inward_and_dx_must_be_a_pixel_multiple(tmp_path):
+    write_asc(tmp_path / "a.asc", ground(), DX / K)
+    r = read_raster(tmp_path / "a.asc")
+    grid, _ = grid_over(r, DX, (X0 + 101.0, Y0 + 52.0, X0 + 400.0, Y0 + 300.0))
+    assert grid.x0 == pytest.approx(X0 + 105.0 - DX)
+    assert int(grid.mask.sum()) == 29 * 24
+    with pytest.raises(ValueError):
+        grid_over(r, 7.0)
+    with pytest.raises(FileNotFoundError):
+        read_raster(tmp_path / "none.asc")
+    (tmp_path / "g.tif").write_bytes(b"II*\x00")
+    with pytest.raises(ValueError):
+        read_raster(tmp_path / "g.tif")
+
+
+# ---- lines
+
+
+def _grid(ny: int = 40, nx: int = 50) -> TileGrid:
+    return TileGrid.from_mask(np.ones((ny, nx), bool), 0.0, ny * 10.0, 10.0)
+
+
+def test_a_line_along_the_edge_opens_the_edge_cells():
+    g = _grid()
+    jj, ii = cells_along(g, [(505.0, 400.0), (505.0, 0.0)], samples=4, snap_m=150.0)
+    assert set(ii.tolist()) == {50} and np.array_equal(np.unique(jj), np.arange(1, 41))
+
+
+def test_a_polygon_takes_the_cells_whose_centre_it_holds():
+    g = _grid()
+    m = inside(g, [(0.0, 400.0), (100.0, 400.0), (100.0, 300.0), (0.0, 300.0)])
+    assert m.sum() == 100 and m[1:11, 1:11].all()
+    tri = inside(g, [(0.0, 0.0), (200.0, 0.0), (0.0, 200.0)])
+    j, i = np.nonzero(tri)
+    x, y = g.xy_of(j, i)
+    assert np.all(x + y < 200.0) and tri.sum() == 190
+
+
+def test_line_faces_follow_the_line_with_the_discharge_to_its_right():
+    g = _grid()
+    # drawn south to north on x = 200 m: west faces of column 21 (x 200..210 m), positive to the east
+    xi, xs, yi, ys = line_faces(g, [(200.0, 0.0), (200.0, 400.0)])
+    assert yi.size == 0 and xi.size == 40 and np.all(xs == 1.0)
+    xi2, xs2, _, _ = line_faces(g, [(200.0, 400.0), (200.0, 0.0)])
+    assert np.array_equal(np.sort(xi2), np.sort(xi)) and np.all(xs2 == -1.0)
+    # drawn west to east on y = 200 m: north faces of row 21, positive to the south
+    xi, xs, yi, ys = line_faces(g, [(0.0, 200.0), (500.0, 200.0)])
+    assert xi.size == 0 and yi.size == 50 and np.all(ys == 1.0)
+    # a diagonal is a staircase of both kinds with one face per lattice step
+    xi, xs, yi, ys = line_faces(g, [(0.0, 0.0), (300.0, 300.0)])
+    assert xi.size == 30 and yi.size == 30
+
+
+# ---- storms
+
+
+def test_storm_curves():
+    from wadi2d_amd.run import storm_curve
+
+    def cfg(**storm):
+        return jobmod.RunConfig.model_validate({"storm": storm})
+
+    u = storm_curve(cfg(kind="uniform", depth_mm=60.0, duration_min=60.0, arf_factor=0.5))
+    assert u(0.0) == 0.0 and u(1800.0) == pytest.approx(15.0) and u(7200.0) == pytest.approx(30.0)
+    h = storm_curve(cfg(kind="hyetograph", hyetograph_min=[10.0, 20.0], hyetograph_mm=[5.0, 15.0]))
+    assert h(300.0) == pytest.approx(2.5) and h(900.0) == pytest.approx(12.5) and h(5000.0) == pytest.approx(20.0)
+    p = storm_curve(cfg(kind="hypothetical", depth_mm=100.0, duration_min=100.0, time_percent=[50.0, 100.0],
+                        depth_percent=[80.0, 100.0]))  # fmt: skip
+    assert p(1500.0) == pytest.approx(40.0) and p(4500.0) == pytest.approx(90.0) and p(9000.0) == pytest.approx(100.0)
+    assert storm_curve(cfg(kind="none")) is None
+
+
+# ---- exit codes without a device
+
+
+def _no_device() -> bool:
+    from wadi2d_amd.solver.hip_backend import device_count
+
+    return device_count() == 0
+
+
+def test_a_run_without_a_device_ends_in_exit_4_with_an_error_event(job_folder):
+    if not _no_device():
+        pytest.skip("a HIP device is present")
+    from wadi2d_amd.run import run_job
+
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "out")
+    assert s.exit_code == 4
+    events = [json.loads(x) for x in (job_folder / "out" / "progress.jsonl").read_text().splitlines()]
+    assert events[-1]["ev"] == "error" and events[-1]["code"] == 4
+    assert (job_folder / "out" / "config.resolved.json").exists()
+
+
+def test_the_command_line_returns_the_protocol_codes(job_folder):
+    cmd = [sys.executable, "-m", "wadi2d_amd", "run"]
+    r = subprocess.run([*cmd, str(job_folder / "none.json"), "--out", str(job_folder / "o")], capture_output=True,
+                       text=True, cwd=ROOT)  # fmt: skip
+    assert r.returncode == 3 and json.loads(r.stderr)["code"] == 3
+    d = job_dict()
+    d["terrain"]["objects"] = [{"kind": "wall"}]
+    (job_folder / "bad.json").write_text(json.dumps(d), encoding="utf-8")
+    r = subprocess.run([*cmd, str(job_folder / "bad.json"), "--out", str(job_folder / "o")], capture_output=True,
+                       text=True, cwd=ROOT)  # fmt: skip
+    assert r.returncode == 2
+    r = subprocess.run([*cmd, str(job_folder / "job.json"), "--out", str(job_folder / "o"), "--dry-run"],
+                       capture_output=True, text=True, cwd=ROOT)  # fmt: skip
+    assert r.returncode == 0 and json.loads(r.stdout)["cells"] == NY * NX
+    if _no_device():
+        r = subprocess.run([*cmd, str(job_folder / "job.json"), "--out", str(job_folder / "o")], capture_output=True,
+                           text=True, cwd=ROOT)  # fmt: skip
+        assert r.returncode == 4
+
+
+# ---- runs on the device
+
+
+def _oracle_run(cfg):
+    """The same job through the oracle, in the window loop of run._run (run.py:1712-1739)."""
+    from oracle.oracle import Ledger, OracleModel
+    from oracle.oracle import Numerics as ONumerics
+    from wadi2d_amd.engine.forcing import Rain, boundary_set
+    from wadi2d_amd.run import boundary_list, build_grid, initial_depth, storm_curve
+    from wadi2d_amd.solver.state import from_state
+
+    tables = build_grid(cfg)
+    bset = boundary_set(boundary_list(cfg, tables), tables)
+    curve = storm_curve(cfg)
+    rain = Rain(tables, bset, curve if curve is not None else (lambda t: 0.0))
+    m = OracleModel(tables, ONumerics.like(cfg.numerics), prec=4, threads=4)
+    m.set_depth(initial_depth(cfg, tables))
+    if bset.size:
+        m.set_boundary(bset.cells)
+    ledger = Ledger(m, tables.grid.mask & ~bset.mask, bset.xfaces, bset.xsign, bset.yfaces, bset.ysign)
+    t, dry, end = 0.0, False, cfg.run.duration_s
+    while t < end - 1e-9:
+        t_next = min(t + cfg.run.source_every_s, end)
+        rate, vol = rain.rate(t, t_next)
+        if vol > 0.0 or not dry:
+            m.set_source_cells(rain.state, rate)
+        dry = vol <= 0.0
+        ledger.open_window(t, t_next, vol, 0.0)
+        m.advance(t_next, ledger.on_step)
+        ledger.settle(t_next)
+        t = t_next
+    return from_state(tables.grid, ledger.dmax), ledger.accounts(t), m.stepper.nsteps
+
+
+@pytest.mark.gpu
+def test_a_job_runs_end_to_end_bit_for_bit_the_oracle(job_folder):
+    from wadi2d_amd.run import run_job
+
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "out", backend="hip-exact")
+    print(f"run: exit {s.exit_code} steps {s.steps} mass {s.balance.relative:.2e} peaks {s.peaks}", flush=True)
+    assert s.exit_code == 0, s.message
+    dmax, bal, steps = _oracle_run(cfg)
+    got = np.load(job_folder / "out" / "results" / "maxima" / "depth.npy")
+    assert s.steps == steps
+    assert np.array_equal(got, dmax.astype(np.float32))
+    assert (got > 0.05).sum() > 500  # the reservoir and the storm wet the domain
+    assert abs(s.balance.relative) <= 1e-6 and abs(s.balance.relative - bal.relative) <= 1e-9
+    events = [json.loads(x) for x in (job_folder / "out" / "progress.jsonl").read_text().splitlines()]
+    kinds = [e["ev"] for e in events]
+    assert kinds[0] in ("warn", "start") and "start" in kinds and kinds[-1] == "done"
+    assert events[-1]["code"] == 0 and events[-1]["steps"] == s.steps
+    assert (job_folder / "out" / "series" / "lines_gap.csv").exists()
+    maps = sorted((job_folder / "out" / "results" / "maps").iterdir())
+    assert len(maps) >= 2 and (maps[0] / "depth.npy").exists()
+
+
+@pytest.mark.gpu
+def test_a_stopped_and_resumed_run_is_bit_for_bit_the_unbroken_one(job_folder):
+    from wadi2d_amd.run import run_job
+
+    cfg = jobmod.load_job(job_folder / "job.json")
+    whole = run_job(cfg, job_folder / "whole", backend="hip-exact")
+    first = run_job(cfg, job_folder / "split", backend="hip-exact", stop_at_s=240.0)
+    assert first.exit_code == 6 and (job_folder / "split" / "checkpoint.npz").exists()
+    assert first.t_end == pytest.approx(240.0)
+    second = run_job(cfg, job_folder / "split", backend="hip-exact", resume=True)
+    print(f"whole {whole.steps} steps, split {first.steps} + resumed to {second.steps}", flush=True)
+    assert whole.exit_code == 0 and second.exit_code == 0
+    assert second.steps == whole.steps
+    for name in ("depth", "mean_depth", "speed", "hazard", "arrival"):
+        a = np.load(job_folder / "whole" / "results" / "maxima" / f"{name}.npy")
+        b = np.load(job_folder / "split" / "results" / "maxima" / f"{name}.npy")
+        assert np.array_equal(a, b), name
+    assert second.balance.relative == whole.balance.relative
+    for f in ("lines_gap.csv", "points_p1.csv"):
+        assert (job_folder / "whole" / "series" / f).read_text() == (job_folder / "split" / "series" / f).read_text()
+    # a checkpoint of one job does not continue another
+    other = cfg.model_copy(update={"terrain": cfg.terrain.model_copy(update={"manning": 0.05})})
+    s = run_job(other, job_folder / "split", backend="hip-exact", resume=True)
+    assert s.exit_code == 2

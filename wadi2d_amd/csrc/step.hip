@@ -2143,6 +2143,15 @@ int w2d_tally_volume(w2d_handle* H, double* out) {
     return sum_doubles(H, H->t_total, H->n_tally, out);
 }
 
+int w2d_tally_restore(w2d_handle* H, const double* totals, int32_t n) {
+    if (!H || n != H->n_tally || (n && !totals)) return fail(1, "tally_restore: bad arguments (one total per face)");
+    if (n) {
+        HIP_TRY(hipMemcpyAsync(H->t_total, totals, n * sizeof(double), hipMemcpyHostToDevice, H->stream));
+        HIP_TRY(hipStreamSynchronize(H->stream));
+    }
+    return 0;
+}
+
 int w2d_tally_flow(w2d_handle* H, double* out) {
     if (!H || !out) return fail(1, "tally_flow: bad arguments");
     if (!H->n_tally) {
@@ -2223,6 +2232,20 @@ int w2d_maxima_get(w2d_handle* H, int32_t which, void* host, int64_t n) {
         expand(H, (const double*)own.data(), (double*)host);
     else
         expand(H, (const float*)own.data(), (float*)host);
+    return 0;
+}
+
+int w2d_maxima_put(w2d_handle* H, int32_t which, const void* host, int64_t n) {
+    if (!H || !host || which < 0 || which > 4 || !H->maxima[which] || n != (int64_t)H->ntiles * STRIDE2)
+        return fail(1, "maxima_put: bad arguments (call maxima_set first)");
+    const size_t es = esize(H);
+    std::vector<char> own(H->ncell * es);
+    if (es == 8)
+        compact(H, (const double*)host, (double*)own.data());
+    else
+        compact(H, (const float*)host, (float*)own.data());
+    HIP_TRY(hipMemcpyAsync(H->maxima[which], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
     return 0;
 }
 

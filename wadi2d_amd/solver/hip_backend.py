@@ -73,6 +73,8 @@ _SIGS = {
     "w2d_maxima_set": [VP, D, D],
     "w2d_maxima_update": [VP, D],
     "w2d_maxima_get": [VP, I32, VP, I64],
+    "w2d_maxima_put": [VP, I32, VP, I64],
+    "w2d_tally_restore": [VP, VP, I32],
     "w2d_storage_set": [VP, VP, I64],
     "w2d_storage": [VP, ctypes.POINTER(D)],
     "w2d_synchronize": [VP],
@@ -285,6 +287,11 @@ class HipBackend:
             check(api().w2d_tally_totals(self.h, out.ctypes.data))
         return out
 
+    def tally_restore(self, totals: np.ndarray) -> None:
+        """Saved per-face totals back, as ``Tally.restore`` (engine/maxima.py:189-192) for a checkpoint's resume."""
+        t = _c(totals, np.float64)
+        check(api().w2d_tally_restore(self.h, t.ctypes.data if t.size else None, int(t.size)))
+
     def tally_flow(self) -> float:
         out = D(0.0)
         check(api().w2d_tally_flow(self.h, ctypes.byref(out)))
@@ -300,6 +307,13 @@ class HipBackend:
         out = np.zeros(self.n, self.dtype)
         check(api().w2d_maxima_get(self.h, int(which), out.ctypes.data, self.n))
         return out
+
+    def maxima_put(self, which: int, a: np.ndarray) -> None:
+        """Saved maxima back (state layout), as ``Maxima.restore`` (engine/maxima.py:136-138)."""
+        src = _c(a, self.dtype)
+        if src.size != self.n:
+            raise ValueError(f"maxima_put: {src.size} values for {self.n} state entries")
+        check(api().w2d_maxima_put(self.h, int(which), src.ctypes.data, self.n))
 
     def source_weights(self, index: np.ndarray, weight: np.ndarray) -> None:
         idx, w = _c(index, np.int64), _c(weight, np.float64)
