@@ -36,7 +36,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # algorithmic bytes per own cell (float32), each array counted once per cell per step
-FACE_BYTES = 144  # h,d,u,v,mx,my,qx,qy 32 + zcx,zcy 8 + face statics 48 + two table pair reads 32 + 6 writes 24
+FACE_BYTES = 132  # h,d,u,v,qx,qy 24 + previous h 4 + zcx,zcy 8 + face statics 48 + two table pairs 32 + 4 writes 16
 CELL_BYTES = 52   # h,qx1,qy1,res,src 20 + Hc 4 + level pair 8 + writes qx,qy,h,d,res 20
 STEP_BYTES = 176  # the whole step as one pass: 140 read + 36 written
 

@@ -641,6 +641,10 @@ struct StepArgs {
     // the learned closure's per-face corrections (null when it is off)
     const float2* corr_x;
     const float2* corr_y;
+    // face momentum: fresh says mx, my hold it (set from the host); else it is the last step's Abar * u, Abar from
+    // the mean depths that step read (h_old), recomputed here (a step never stores it)
+    const R* h_old;
+    int fresh;
 };
 
 // Both own faces of every own cell of one active tile per workgroup of NTH threads (faces.cu:244-271).
@@ -678,6 +682,8 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1};
     const int i = threadIdx.x & (TILE - 1);
     const bool want_wave = want_run != 0;
+    const bool fresh = A.fresh != 0;
+    const R dxr = R(A.P.dx);
     const R dt = dt_run;
     R w = R(0);
 #pragma unroll
@@ -692,17 +698,26 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
             cx = A.corr_x[c];
             cy = A.corr_y[c];
         }
-        const R wx = face<R, FAST, LEARNED>(fx, L, ld_nt(A.s.mx + c), ld_nt(A.st.sx + c), R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
+        // the face momenta the last step left: Abar * u with Abar = 0.5 (hL + hR) dx from the depths it read, the
+        // same operations on the same operands as its own product, so the same bits as a stored copy
+        R mxo, myo;
+        if (fresh) {
+            mxo = A.s.mx[c];
+            myo = A.s.my[c];
+        } else {
+            const R ho = ld_nt(A.h_old + c);
+            mxo = R(0.5) * (A.h_old[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
+            myo = R(0.5) * (A.h_old[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
+        }
+        const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
                                             A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
                                             A.s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
         st_nt(un, A.s.u1 + c);
-        st_nt(mn, A.s.mx1 + c);
         st_nt(qn, A.s.qx1 + c);
-        const R wy = face<R, FAST, LEARNED>(fy, L, ld_nt(A.s.my + c), ld_nt(A.st.sy + c), R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
+        const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
                                             A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
                                             A.s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
         st_nt(un, A.s.v1 + c);
-        st_nt(mn, A.s.my1 + c);
         st_nt(qn, A.s.qy1 + c);
         const R m = wx > wy ? wx : wy;
         w = m > w ? m : w;
@@ -1002,6 +1017,31 @@ __global__ void copy_tiles_kernel(const int* __restrict__ tiles, int n, const R*
     for (int p = threadIdx.x; p < TILE2; p += NT) b[base + p] = a[base + p];
 }
 
+// The face momenta mx, my of the listed tiles (all active tiles: n, or glob->nact in a device-resident run) from the
+// velocities and the mean depths the last step read, Abar * u as the face kernel recomputes them, into both buffers
+// (a tile off the list keeps one momentum whichever buffer is current).
+template <class R>
+__global__ __launch_bounds__(NT) void momentum_kernel(const int* __restrict__ tiles, int n,
+                                                      const RunGlobals* __restrict__ glob,
+                                                      const int* __restrict__ nbr, const R* __restrict__ h_old,
+                                                      const R* __restrict__ u, const R* __restrict__ v,
+                                                      R* __restrict__ mx, R* __restrict__ my, R* __restrict__ mx2,
+                                                      R* __restrict__ my2, R dx) {
+    if ((int)blockIdx.x >= (glob ? glob->nact : n)) return;
+    const int t = tiles[blockIdx.x];
+    const int tw = nbr[(size_t)t * 8 + 3], tn = nbr[(size_t)t * 8 + 1];
+    for (int p = threadIdx.x; p < TILE2; p += NT) {
+        const int j = p / TILE, i = p - (p / TILE) * TILE;
+        const size_t c = (size_t)t * TILE2 + p;
+        const size_t w = i > 0 ? c - 1 : (size_t)tw * TILE2 + j * TILE + (TILE - 1);
+        const size_t q = j > 0 ? c - TILE : (size_t)tn * TILE2 + (TILE - 1) * TILE + i;
+        const R a = R(0.5) * (h_old[w] + h_old[c]) * dx * u[c];
+        const R b = R(0.5) * (h_old[q] + h_old[c]) * dx * v[c];
+        mx[c] = a, mx2[c] = a;
+        my[c] = b, my2[c] = b;
+    }
+}
+
 // Max of the active tiles' wave speeds into out[0], by one workgroup.
 template <class R>
 __global__ __launch_bounds__(1024) void wave_max_kernel(const int* __restrict__ active, int n,
@@ -1279,10 +1319,26 @@ template <class R, bool FAST>
 __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, R* __restrict__ h,
                             R* __restrict__ h_other, R* __restrict__ om, R* const* __restrict__ faces_cur,
                             R* const* __restrict__ faces_other, const float* __restrict__ hc,
-                            const float2* __restrict__ ctab, size_t ncell, int last) {
+                            const float2* __restrict__ ctab, size_t ncell, int last, R dx, int fresh) {
     for (int q = 0; q < nw; ++q) {
         const size_t c = (size_t)L.woken[b.tile0 + q] * TILE2 + threadIdx.x;
         om[c] = cell_plan<R, FAST>(h[c], R(hc[c]), ctab + c, ncell, last);
+    }
+    // a leaving tile's face momentum is stored (both buffers) from the depths the last step read, before those
+    // depth buffers are made one; a stored momentum (set from the host since) is already there
+    if (nl && !fresh) {
+        for (int q = 0; q < nl; ++q) {
+            const int t = L.leaving[b.tile0 + q];
+            const int p = threadIdx.x, j = p / TILE, i = p - (p / TILE) * TILE;
+            const size_t c = (size_t)t * TILE2 + p;
+            const size_t w = i > 0 ? c - 1 : (size_t)L.nbr[(size_t)t * 8 + 3] * TILE2 + j * TILE + (TILE - 1);
+            const size_t n = j > 0 ? c - TILE : (size_t)L.nbr[(size_t)t * 8 + 1] * TILE2 + (TILE - 1) * TILE + i;
+            const R a = R(0.5) * (h_other[w] + h_other[c]) * dx * faces_cur[0][c];
+            const R e = R(0.5) * (h_other[n] + h_other[c]) * dx * faces_cur[1][c];
+            faces_cur[2][c] = a, faces_other[2][c] = a;
+            faces_cur[3][c] = e, faces_other[3][c] = e;
+        }
+        __syncthreads();
     }
     for (int q = 0; q < nl; ++q) {
         const size_t c = (size_t)L.leaving[b.tile0 + q] * TILE2 + threadIdx.x;
@@ -1303,7 +1359,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
                                                           R* const* __restrict__ faces_other,
                                                           const float* __restrict__ hc,
                                                           const float2* __restrict__ ctab, size_t ncell, int last,
-                                                          int flush_only) {
+                                                          R dx, int fresh, int flush_only) {
     static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
     __shared__ int wave_cnt[3][PRE_THREADS / 64];
     __shared__ int any;
@@ -1312,7 +1368,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
     bool changed = false;
     if (b.check) {
         rebuild(b, 2, L, wave_cnt, nw, nl);
-        transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+        transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last, dx, fresh);
         if (threadIdx.x == 0) b.check = 0;
         changed = true;
     }
@@ -1325,7 +1381,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         __syncthreads();
         if (any) {
             rebuild(b, b.rebuild, L, wave_cnt, nw, nl);
-            transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last);
+            transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last, dx, fresh);
             changed = true;
         }
     }
@@ -1504,6 +1560,9 @@ struct w2d_handle {
     int hcur = 0;
     void* uv[4][2] = {};  // u, v, mx, my: [field][buffer]
     int ucur = 0;
+    // the face momentum: steps do not store it (the face kernel recomputes it from u and the depths the last step
+    // read); m_fresh says whether mx, my hold it, as after set_field or materialize; it goes to the kernels by value
+    int m_fresh = 1;
     void *d = nullptr, *qx = nullptr, *qy = nullptr, *qx1 = nullptr, *qy1 = nullptr, *res = nullptr,
          *src = nullptr, *om = nullptr;
     void* wave = nullptr;
@@ -1654,6 +1713,8 @@ StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     A.want_wave = want_wave;
     A.wave = (R*)H->wave;
     A.wet = H->wet;
+    A.h_old = (const R*)H->h[1 - H->hcur];
+    A.fresh = H->m_fresh;
     return A;
 }
 
@@ -1724,6 +1785,35 @@ int launch_step(w2d_handle* H, double dt, int want_wave) {
     }
     return 0;
 }
+
+// Stores the face momentum of every listed tile (both buffers) if steps have run since it was last stored, so that
+// whatever then changes u, v, h or the list, mx and my keep the value the last step left.
+int materialize(w2d_handle* H) {
+    if (H->m_fresh) return 0;
+    const bool run = H->run_m > 0;
+    const int blocks = run ? H->ntiles : H->nact;
+    if (blocks > 0) {
+        const int o = 1 - H->ucur;
+        if (H->prec == 8)
+            momentum_kernel<double><<<blocks, NT, 0, H->stream>>>(
+                H->active, H->nact, run ? H->run_glob : nullptr, H->nbr, (const double*)H->h[1 - H->hcur],
+                (const double*)H->uv[0][H->ucur], (const double*)H->uv[1][H->ucur], (double*)H->uv[2][H->ucur],
+                (double*)H->uv[3][H->ucur], (double*)H->uv[2][o], (double*)H->uv[3][o], H->P.dx);
+        else
+            momentum_kernel<float><<<blocks, NT, 0, H->stream>>>(
+                H->active, H->nact, run ? H->run_glob : nullptr, H->nbr, (const float*)H->h[1 - H->hcur],
+                (const float*)H->uv[0][H->ucur], (const float*)H->uv[1][H->ucur], (float*)H->uv[2][H->ucur],
+                (float*)H->uv[3][H->ucur], (float*)H->uv[2][o], (float*)H->uv[3][o], (float)H->P.dx);
+        HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    H->m_fresh = 1;
+    return 0;
+}
+
+// Fields that hold the face momentum, and fields whose change would move the momentum the last step left.
+bool momentum_store(int f) { return f == W2D_MX || f == W2D_MY || f == W2D_MX1 || f == W2D_MY1; }
+bool momentum_field(int f) { return momentum_store(f) || f == W2D_H || f == W2D_U || f == W2D_V; }
 
 int run_step(w2d_handle* H, double dt, int want_wave) {
     if (H->prec == 8) return launch_step<double, false>(H, dt, want_wave);
@@ -1908,6 +1998,8 @@ int w2d_set_field(w2d_handle* H, int32_t f, const void* host, int64_t n) {
     if (!H || !host || n != (int64_t)H->ntiles * STRIDE2) return fail(1, "set_field: bad arguments");
     void* dst = field_ptr(H, f);
     if (!dst) return fail(1, "set_field: the device keeps no copy of this field");
+    int s;
+    if (momentum_field(f) && (s = materialize(H))) return s;
     const size_t es = esize(H);
     std::vector<char> own(H->ncell * es);
     if (es == 8)
@@ -1918,6 +2010,12 @@ int w2d_set_field(w2d_handle* H, int32_t f, const void* host, int64_t n) {
     std::memset(own.data(), 0, TILE2 * es);
     HIP_TRY(hipMemcpyAsync(dst, own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
     if (f == W2D_H) HIP_TRY(hipMemcpyAsync(H->h[1 - H->hcur], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
+    // the momentum is one value per face whichever buffer is current (no step stores a second one)
+    if (f == W2D_MX || f == W2D_MY || f == W2D_MX1 || f == W2D_MY1) {
+        const int k = (f == W2D_MX || f == W2D_MX1) ? 2 : 3;
+        HIP_TRY(hipMemcpyAsync(H->uv[k][0], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
+        HIP_TRY(hipMemcpyAsync(H->uv[k][1], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
+    }
     HIP_TRY(hipStreamSynchronize(H->stream));
     return 0;
 }
@@ -1926,6 +2024,8 @@ int w2d_get_field(w2d_handle* H, int32_t f, void* host, int64_t n) {
     if (!H || !host || n != (int64_t)H->ntiles * STRIDE2) return fail(1, "get_field: bad arguments");
     void* src = field_ptr(H, f);
     if (!src) return fail(1, "get_field: the device keeps no copy of this field (the positivity ratio lives in LDS)");
+    int s;
+    if (momentum_store(f) && (s = materialize(H))) return s;
     const size_t es = esize(H);
     std::vector<char> own(H->ncell * es);
     HIP_TRY(hipMemcpyAsync(own.data(), src, own.size(), hipMemcpyDeviceToHost, H->stream));
@@ -1941,6 +2041,8 @@ int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, vo
     if (!H || (n && (!index || !out))) return fail(1, "read_cells: bad arguments");
     void* src = field_ptr(H, f);
     if (!src) return fail(1, "read_cells: the device keeps no copy of this field");
+    int s;
+    if (momentum_store(f) && (s = materialize(H))) return s;
     const size_t es = esize(H);
     for (int64_t m = 0; m < n; ++m) {
         const int64_t c = own_of(H, index[m]);
@@ -1964,6 +2066,9 @@ int w2d_set_active(w2d_handle* H, const int32_t* tiles, int32_t n) {
     for (int t : H->active_host)
         if (!now[t]) leaving.push_back(t);
     if (!leaving.empty()) {
+        // the momentum of every listed tile is stored first: a leaving tile's depth buffers are about to be made one
+        int s;
+        if ((s = materialize(H))) return s;
         HIP_TRY(hipMemcpyAsync(H->tiles_scratch, leaving.data(), leaving.size() * sizeof(int), hipMemcpyHostToDevice, H->stream));
         if (H->prec == 8)
             copy_tiles_kernel<double><<<(int)leaving.size(), NT, 0, H->stream>>>(H->tiles_scratch, (int)leaving.size(),
@@ -2020,6 +2125,7 @@ int w2d_step(w2d_handle* H, double dt, int32_t want_wave, int32_t want_wet) {
     if (s) return s;
     H->hcur = 1 - H->hcur;
     H->ucur = 1 - H->ucur;
+    H->m_fresh = 0;
     return 0;
 }
 
@@ -2439,7 +2545,7 @@ int enqueue_pre(w2d_handle* H, int flush_only) {
     R* const* fc = (R* const*)(H->run_faces_ptrs + 8 * H->ucur);  // current u, v, mx, my, then the others
     pre_kernel<R, FAST><<<H->run_m, PRE_THREADS, 0, H->stream>>>(
         H->run_mem, H->run_glob, H->run_scales, H->run_num, run_lists(H), (R*)H->h[H->hcur], (R*)H->h[1 - H->hcur],
-        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, flush_only);
+        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, (R)H->P.dx, H->m_fresh, flush_only);
     HIP_TRY(hipGetLastError());
     gather_kernel<<<1, PRE_THREADS, 0, H->stream>>>(H->run_mem, H->run_m, H->run_glob, H->run_seg, H->active);
     HIP_TRY(hipGetLastError());
@@ -2475,6 +2581,7 @@ int enqueue_run_step(w2d_handle* H) {
     HIP_TRY(hipGetLastError());
     H->hcur = 1 - H->hcur;
     H->ucur = 1 - H->ucur;
+    H->m_fresh = 0;
     return 0;
 }
 
@@ -2636,6 +2743,11 @@ int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
     if (!H || !H->run_m || nsteps < 0) return fail(1, "run_steps: call run_setup first");
     if (graph_steps > 0 && graph_steps % 2) return fail(1, "run_steps: a graph holds an even number of steps");
     int done = 0, s;
+    // a momentum stored from the host is read by one step launched on its own: captured steps take it as recomputed
+    if (H->m_fresh && nsteps > 0) {
+        if ((s = run_step_any(H))) return s;
+        ++done;
+    }
     // HIP cannot time events recorded inside a captured graph: timed steps are launched one by one
     if (graph_steps > 0 && !H->timing) {
         if (H->graph && H->graph_steps != graph_steps) {

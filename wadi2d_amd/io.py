@@ -114,9 +114,10 @@ def write_series_csv(path: str | Path, t, columns: dict[str, Any]) -> Path:
     return out
 
 
-SAVED_FIELDS = ("h", "d", "u", "v", "mx", "my", "qx", "qy", "res", "om", "u1", "v1", "mx1", "my1", "qx1", "qy1")
-"""The state, the residuals, the wet share, and the other face buffers (io/checkpoint.py:17-19 saves the first ten;
-the second buffers make a resume bit for bit the unbroken run on tiles that sleep)."""
+SAVED_FIELDS = ("h", "d", "u", "v", "mx", "my", "qx", "qy", "res", "om", "u1", "v1", "qx1", "qy1")
+"""The state, the residuals, the wet share, and the second velocity and discharge buffers (io/checkpoint.py:17-19
+saves the first ten; the second buffers make a resume bit for bit the unbroken run on tiles that sleep; the device
+keeps one face momentum, so mx1 and my1 are mx and my)."""
 
 
 def save_checkpoint(path: str | Path, payload: dict[str, Any]) -> Path:
