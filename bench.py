@@ -36,7 +36,11 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 # algorithmic bytes per own cell (float32), each array counted once per cell per step
-FACE_BYTES = 132  # h,d,u,v,qx,qy 24 + previous h 4 + zcx,zcy 8 + face statics 48 + two table pairs 32 + 4 writes 16
+# The face step's algorithmic bytes per own cell, its share of STEP_BYTES: reads h,d,u,v,mx,my,qx,qy 32 + zcx,zcy 8
+# + face statics 48 + two table row pairs 32, writes u,v,mx,my,qx,qy 24. The kernel moves 124 of them: it recomputes
+# mx,my from u and the previous step's depths (4 B read instead of 8 read and 8 written) and carries the crest flag in
+# the sign of the table top (the crest coefficient read only for a crest face).
+FACE_BYTES = 144
 CELL_BYTES = 52   # h,qx1,qy1,res,src 20 + Hc 4 + level pair 8 + writes qx,qy,h,d,res 20
 STEP_BYTES = 176  # the whole step as one pass: 140 read + 36 written
 

@@ -146,6 +146,11 @@ __device__ __forceinline__ float4 ld_nt(const float4* p) {
     const v4 v = __builtin_nontemporal_load(reinterpret_cast<const v4*>(p));
     return make_float4(v.x, v.y, v.z, v.w);
 }
+__device__ __forceinline__ float2 ld_nt(const float2* p) {
+    typedef float v2 __attribute__((ext_vector_type(2)));
+    const v2 v = __builtin_nontemporal_load(reinterpret_cast<const v2*>(p));
+    return make_float2(v.x, v.y);
+}
 template <class T>
 __device__ __forceinline__ void st_nt(T v, T* p) { __builtin_nontemporal_store(v, p); }
 
@@ -256,7 +261,8 @@ struct Statics {
     const float4* ftaby;
     const float4* sx;         // (fl, fr, gl, gr) of x faces
     const float4* sy;
-    const float4* ec;         // (efx, cwx, efy, cwy)
+    const float2* es;         // (efx, efy) table tops, negative for a face with a crest coefficient
+    const float2* cwv;        // (cwx, cwy), read only for such a face
     const float* zcx;         // ncell, own layout; a neighbour tile's value is read from that tile
     const float* zcy;
     float4* fcx;              // row caches of the face tables and their row tags (255: empty)
@@ -691,7 +697,10 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
-        const float4 e = ld_nt(A.st.ec + c);
+        const float2 e = ld_nt(A.st.es + c);
+        const R efx = R(fabsf(e.x)), efy = R(fabsf(e.y));
+        const R cwx = signbit(e.x) ? R(A.st.cwv[c].x) : R(0);
+        const R cwy = signbit(e.y) ? R(A.st.cwv[c].y) : R(0);
         R un, mn, qn;
         float2 cx{1.f, 0.f}, cy{1.f, 0.f};
         if constexpr (LEARNED) {
@@ -709,12 +718,12 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
             mxo = R(0.5) * (A.h_old[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
             myo = R(0.5) * (A.h_old[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
         }
-        const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), R(e.x), R(e.y), A.st.zcx, A.st.ftabx + c,
+        const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), efx, cwx, A.st.zcx, A.st.ftabx + c,
                                             A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
                                             A.s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
         st_nt(un, A.s.u1 + c);
         st_nt(qn, A.s.qx1 + c);
-        const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), R(e.z), R(e.w), A.st.zcy, A.st.ftaby + c,
+        const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), efy, cwy, A.st.zcy, A.st.ftaby + c,
                                             A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
                                             A.s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
         st_nt(un, A.s.v1 + c);
@@ -1572,7 +1581,8 @@ struct w2d_handle {
     // statics
     float* hc = nullptr;
     float2* ctab = nullptr;
-    float4 *ftabx = nullptr, *ftaby = nullptr, *sx = nullptr, *sy = nullptr, *ec = nullptr;
+    float4 *ftabx = nullptr, *ftaby = nullptr, *sx = nullptr, *sy = nullptr;
+    float2 *es = nullptr, *cwv = nullptr;
     float *zcx = nullptr, *zcy = nullptr;
     float4 *fcx = nullptr, *fcy = nullptr;
     float2* cc = nullptr;
@@ -1693,7 +1703,8 @@ StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     A.st.ftaby = H->ftaby;
     A.st.sx = H->sx;
     A.st.sy = H->sy;
-    A.st.ec = H->ec;
+    A.st.es = H->es;
+    A.st.cwv = H->cwv;
     A.st.zcx = H->zcx;
     A.st.zcy = H->zcy;
     A.st.fcx = H->face_cache ? H->fcx : nullptr;
@@ -1950,15 +1961,23 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
         if ((s = upload(H, axis ? &H->ftaby : &H->ftabx, f4.data(), f4.size()))) return bail(s);
     }
     {
-        std::vector<float4> sx(n), sy(n), ec(n);
+        std::vector<float4> sx(n), sy(n);
+        std::vector<float2> es(n), cwv(n);
+        // a face's table top carries in its sign whether the face has a crest coefficient (exact: a sign flip);
+        // the coefficient itself is read only for such a face
+        auto flag = [](float ef, float cw) { return cw != 0.0f ? -std::fabs(ef) : std::fabs(ef); };
         for (size_t c = 0; c < n; ++c) {
             sx[c] = make_float4(T->flx[c], T->frx[c], T->glx[c], T->grx[c]);
             sy[c] = make_float4(T->fly[c], T->fry[c], T->gly[c], T->gry[c]);
-            ec[c] = make_float4(T->efx[c], T->cwx[c], T->efy[c], T->cwy[c]);
+            es[c] = make_float2(flag(T->efx[c], T->cwx[c]), flag(T->efy[c], T->cwy[c]));
+            cwv[c] = make_float2(T->cwx[c], T->cwy[c]);
         }
+        for (size_t c = 0; c < n; ++c)
+            if (!(T->efx[c] >= 0.0f) || !(T->efy[c] >= 0.0f)) return bail(fail(1, "create: a face table top is negative or NaN"));
         if ((s = upload(H, &H->sx, sx.data(), n))) return bail(s);
         if ((s = upload(H, &H->sy, sy.data(), n))) return bail(s);
-        if ((s = upload(H, &H->ec, ec.data(), n))) return bail(s);
+        if ((s = upload(H, &H->es, es.data(), n))) return bail(s);
+        if ((s = upload(H, &H->cwv, cwv.data(), n))) return bail(s);
     }
     if ((s = upload(H, &H->hc, T->hc, n))) return bail(s);
     H->nbr_host.assign(T->nbr, T->nbr + (size_t)H->ntiles * 8);
