@@ -602,6 +602,16 @@ __device__ __forceinline__ R block_max(R v, R* red) {
     return m;
 }
 
+// A wave speed (never negative, never NaN: the maxima above skip NaN) as bits whose unsigned order is its order, for
+// an atomic max across workgroups; and back.
+__device__ __forceinline__ unsigned long long wave_bits(float v) { return (unsigned long long)__float_as_uint(v); }
+__device__ __forceinline__ unsigned long long wave_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+template <class R>
+__device__ __forceinline__ double wave_value(unsigned long long b) {
+    if constexpr (std::is_same<R, float>::value) return (double)__uint_as_float((unsigned)b);
+    else return __longlong_as_double((long long)b);
+}
+
 // One member of a device-resident run: a model with its own clock, step, source windows and books. A plain run
 // is one member; an ensemble is many, their tiles side by side in one set of arrays (tile0 .. tile1).
 struct Member {
@@ -619,7 +629,8 @@ struct Member {
 
 struct RunGlobals {
     int nact;
-    int changed;  // a member's list changed since the active list was last gathered
+    int changed;   // a member's list changed since the active list was last gathered
+    int pre_done;  // workgroups of the pre kernel that have finished (the last gathers the lists, then clears it)
 };
 
 struct RunNum {
@@ -647,6 +658,8 @@ struct StepArgs {
     // the learned closure's per-face corrections (null when it is off)
     const float2* corr_x;
     const float2* corr_y;
+    // device-resident run: each member's largest wave speed of the step, as wave_bits (null in a protocol step)
+    unsigned long long* member_wave;
     // face momentum: fresh says mx, my hold it (set from the host); else it is the last step's Abar * u, Abar from
     // the mean depths that step read (h_old), recomputed here (a step never stores it)
     const R* h_old;
@@ -669,8 +682,10 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     const int t = A.active[slot];
     R dt_run = A.dt;
     int want_run = A.want_wave;
+    int member = 0;
     if (A.members) {
-        const Member& mb = A.members[A.tile_member[t]];
+        member = A.tile_member[t];
+        const Member& mb = A.members[member];
         if (!mb.stepping) return;
         dt_run = (R)mb.dt_step;
         want_run = mb.refresh;
@@ -733,7 +748,10 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     }
     if (want_wave) {
         const R m = block_max<NTH>(w, red);
-        if (threadIdx.x == 0) A.wave[t] = m;
+        if (threadIdx.x == 0) {
+            A.wave[t] = m;
+            if (A.member_wave) atomicMax(A.member_wave + member, wave_bits(m));
+        }
     }
 }
 
@@ -1358,8 +1376,26 @@ __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, 
     __syncthreads();
 }
 
+// The members' lists end to end, in member (so ascending tile) order, when one of them changed; one workgroup.
+__device__ void gather_lists(const Member* __restrict__ mem, int M, RunGlobals* __restrict__ g,
+                             const int* __restrict__ seg, int* __restrict__ active) {
+    if (!g->changed) return;
+    int off = 0;
+    for (int m = 0; m < M; ++m) {
+        const Member& b = mem[m];
+        for (int k = threadIdx.x; k < b.nlist; k += PRE_THREADS) active[off + k] = seg[b.tile0 + k];
+        off += b.nlist;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        g->nact = off;
+        g->changed = 0;
+    }
+}
+
 // Before a step, one workgroup per member: the check the last step asked for (stepper.py:113-114), the clock and
-// window, the list change those ask for, and the tiles that woke or left.
+// window, the list change those ask for, and the tiles that woke or left; the last workgroup to finish then gathers
+// the members' lists into the active list.
 template <class R, bool FAST>
 __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ mem, RunGlobals* __restrict__ g,
                                                           const double* __restrict__ scales, const RunNum num,
@@ -1368,7 +1404,8 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
                                                           R* const* __restrict__ faces_other,
                                                           const float* __restrict__ hc,
                                                           const float2* __restrict__ ctab, size_t ncell, int last,
-                                                          R dx, int fresh, int flush_only) {
+                                                          R dx, int fresh, unsigned long long* __restrict__ member_wave,
+                                                          int flush_only) {
     static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
     __shared__ int wave_cnt[3][PRE_THREADS / 64];
     __shared__ int any;
@@ -1385,6 +1422,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         if (threadIdx.x == 0) {
             int mine = 0;
             ctl_begin(b, scales, num, mine);
+            if (b.refresh) member_wave[blockIdx.x] = 0ull;
             any = mine;
         }
         __syncthreads();
@@ -1395,24 +1433,18 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         }
     }
     if (changed && threadIdx.x == 0) g->changed = 1;
-}
-
-// The members' lists end to end, in member (so ascending tile) order, when one of them changed.
-__global__ __launch_bounds__(PRE_THREADS) void gather_kernel(const Member* __restrict__ mem, int M,
-                                                             RunGlobals* __restrict__ g, const int* __restrict__ seg,
-                                                             int* __restrict__ active) {
-    if (!g->changed) return;
-    int off = 0;
-    for (int m = 0; m < M; ++m) {
-        const Member& b = mem[m];
-        for (int k = threadIdx.x; k < b.nlist; k += PRE_THREADS) active[off + k] = seg[b.tile0 + k];
-        off += b.nlist;
-    }
+    // the last workgroup to finish gathers the members' lists (every other one has published its own first)
+    __shared__ int is_last;
     __syncthreads();
     if (threadIdx.x == 0) {
-        g->nact = off;
-        g->changed = 0;
+        __threadfence();
+        is_last = atomicAdd(&g->pre_done, 1) == (int)gridDim.x - 1;
     }
+    __syncthreads();
+    if (!is_last) return;
+    __threadfence();
+    gather_lists(mem, (int)gridDim.x, g, L.seg, L.active);
+    if (threadIdx.x == 0) g->pre_done = 0;
 }
 
 // After a step, one workgroup per member: its boundary cells (cells.cu:95-111) and FP64 tally
@@ -1421,7 +1453,8 @@ __global__ __launch_bounds__(PRE_THREADS) void gather_kernel(const Member* __res
 constexpr int POST_THREADS = 1024;  // a member's tiles, boundary cells and tally faces are strided over them
 
 template <class R>
-__global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__ mem, const R* __restrict__ wave,
+__global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__ mem,
+                                                   const unsigned long long* __restrict__ member_wave,
                                                    const int* __restrict__ is_active,
                                                    const double* __restrict__ src_partial, const RunNum num,
                                                    const int* __restrict__ cell, const int* __restrict__ ref,
@@ -1431,7 +1464,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
                                                    const float* __restrict__ sign, const R* __restrict__ qx,
                                                    const R* __restrict__ qy, double* __restrict__ total,
                                                    double* __restrict__ flow) {
-    __shared__ double red_w[POST_THREADS / 64], red_s[POST_THREADS / 64];
+    __shared__ double red_s[POST_THREADS / 64];
     Member& b = mem[blockIdx.x];
     if (!b.stepping) return;
     for (int p = b.b0 + threadIdx.x; p < b.b1; p += blockDim.x) {
@@ -1453,33 +1486,20 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
         total[q] += b.dt_step * f;
         flow[q] = f;
     }
+    // the largest wave speed of the member's tiles, gathered by the face kernel's atomic max
     double w = 0.0, rs = 0.0;
-    if (b.refresh || b.new_window) {
-        for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x) {
-            if (!is_active[t]) continue;
-            if (b.refresh) {
-                const double v = (double)wave[t];
-                w = v > w ? v : w;
-            }
-            if (b.new_window) rs += src_partial[t];
-        }
+    if (b.new_window) {
+        for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x)
+            if (is_active[t]) rs += src_partial[t];
+        for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off);
+        if ((threadIdx.x & 63) == 0) red_s[threadIdx.x >> 6] = rs;
+        __syncthreads();
     }
-    for (int off = 32; off > 0; off >>= 1) {
-        const double o = __shfl_xor(w, off);
-        w = o > w ? o : w;
-        rs += __shfl_xor(rs, off);
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red_w[threadIdx.x >> 6] = w;
-        red_s[threadIdx.x >> 6] = rs;
-    }
-    __syncthreads();
     if (threadIdx.x != 0) return;
-    w = red_w[0];
-    rs = red_s[0];
-    for (int q = 1; q < POST_THREADS / 64; ++q) {
-        w = red_w[q] > w ? red_w[q] : w;
-        rs += red_s[q];
+    if (b.refresh) w = wave_value<R>(member_wave[blockIdx.x]);
+    if (b.new_window) {
+        rs = red_s[0];
+        for (int q = 1; q < POST_THREADS / 64; ++q) rs += red_s[q];
     }
     if (b.new_window) b.rain_window = rs * num.area * (b.window_end - b.window_start);
     const long long before = b.nsteps;
@@ -1616,6 +1636,7 @@ struct w2d_handle {
     int *run_tile_member = nullptr, *run_has_source = nullptr, *run_bnd_tile = nullptr, *run_is_active = nullptr,
         *run_idle = nullptr, *run_flags = nullptr, *run_woken = nullptr, *run_leaving = nullptr, *run_seg = nullptr;
     double *run_scales = nullptr, *run_src_partial = nullptr;
+    unsigned long long* run_wave = nullptr;  // per member, the step's largest wave speed as bits
     void** run_faces_ptrs = nullptr;  // [parity][cur 4, other 4]
     int run_maxima = 0;
     std::vector<double> weight_host;
@@ -2567,9 +2588,7 @@ int enqueue_pre(w2d_handle* H, int flush_only) {
     R* const* fc = (R* const*)(H->run_faces_ptrs + 8 * H->ucur);  // current u, v, mx, my, then the others
     pre_kernel<R, FAST><<<H->run_m, PRE_THREADS, 0, H->stream>>>(
         H->run_mem, H->run_glob, H->run_scales, H->run_num, run_lists(H), (R*)H->h[H->hcur], (R*)H->h[1 - H->hcur],
-        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, (R)H->P.dx, H->m_fresh, flush_only);
-    HIP_TRY(hipGetLastError());
-    gather_kernel<<<1, PRE_THREADS, 0, H->stream>>>(H->run_mem, H->run_m, H->run_glob, H->run_seg, H->active);
+        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, (R)H->P.dx, H->m_fresh, H->run_wave, flush_only);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2582,6 +2601,7 @@ int enqueue_run_step(w2d_handle* H) {
     A.glob = H->run_glob;
     A.src_weight = H->src_weight;
     A.src_partial = H->run_src_partial;
+    A.member_wave = H->run_wave;
     const int nt = H->ntiles;
     R* h_new = (R*)H->h[1 - H->hcur];
     int s;
@@ -2591,7 +2611,7 @@ int enqueue_run_step(w2d_handle* H) {
     if (H->timing && (s = mark(H))) return s;
     if ((s = launch_cells<R, FAST>(H, A, nt))) return s;
     if (H->timing && (s = mark(H))) return s;
-    post_kernel<R><<<H->run_m, POST_THREADS, 0, H->stream>>>(H->run_mem, (const R*)H->wave, H->run_is_active,
+    post_kernel<R><<<H->run_m, POST_THREADS, 0, H->stream>>>(H->run_mem, H->run_wave, H->run_is_active,
                                                      H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
                                                      H->b_d0, H->b_h0, h_new, (R*)H->d, H->t_face, H->t_axis,
                                                      H->t_sign, (R*)H->qx, (R*)H->qy, H->t_total, H->t_flow);
@@ -2736,6 +2756,12 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     g.nact = H->nact;
     HIP_TRY(hipMemcpy(H->run_glob, &g, sizeof(g), hipMemcpyHostToDevice));
     if (!H->run_src_partial && (s = dalloc(H, (void**)&H->run_src_partial, (size_t)nt * sizeof(double)))) return s;
+    if (H->run_wave) {
+        H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_wave), H->owned.end());
+        (void)hipFree(H->run_wave);
+        H->run_wave = nullptr;
+    }
+    if ((s = dalloc(H, (void**)&H->run_wave, (size_t)M * sizeof(unsigned long long)))) return s;
     if (!H->run_faces_ptrs) {
         // [ucur = 0: current u, v, mx, my, then the others][ucur = 1: the same, swapped]
         void* ptrs[16];
