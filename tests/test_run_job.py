@@ -370,3 +370,34 @@ def test_a_stopped_and_resumed_run_is_bit_for_bit_the_unbroken_one(job_folder):
     other = cfg.model_copy(update={"terrain": cfg.terrain.model_copy(update={"manning": 0.05})})
     s = run_job(other, job_folder / "split", backend="hip-exact", resume=True)
     assert s.exit_code == 2
+
+
+@pytest.mark.gpu
+def test_every_boundary_kind_and_storm_kind_runs_and_a_collapsing_step_ends_in_exit_5(job_folder):
+    """A stage and a normal boundary beside the free one, a hyetograph storm, then the same job with a minimum step
+    no step can meet: the run stops with exit 5, an error event and a done event with its partial books
+    (run.py:1749-1757, stepper.py:117-124)."""
+    from wadi2d_amd.run import run_job
+
+    xe, yn = X0 + NX * DX, Y0 + NY * DX
+    d = job_dict(duration=300.0)
+    d["storm"] = {"kind": "hyetograph", "hyetograph_min": [2.0, 4.0], "hyetograph_mm": [10.0, 20.0]}
+    d["boundaries"] = [
+        {"kind": "free", "coordinates": [[xe - 1.0, yn], [xe - 1.0, Y0 + 320.0]], "name": "east top"},
+        {"kind": "normal", "slope": 0.01, "coordinates": [[xe - 1.0, Y0 + 310.0], [xe - 1.0, Y0]], "name": "east"},
+        {"kind": "stage", "stage": 100.5, "coordinates": [[X0 + 1.0, Y0 + 100.0], [X0 + 1.0, Y0]], "name": "west"},
+    ]
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "kinds", backend="hip")
+    print(f"kinds: exit {s.exit_code} steps {s.steps} mass {s.balance.relative:.2e}", flush=True)
+    assert s.exit_code == 0, s.message
+    assert abs(s.balance.relative) <= 1e-5  # the stage boundary's inflow is not booked as inflow here either
+    d["numerics"] = {"dt_min": 1.0}
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "fail", backend="hip")
+    assert s.exit_code == 5 and "below the minimum" in s.message
+    events = [json.loads(x) for x in (job_folder / "fail" / "progress.jsonl").read_text().splitlines()]
+    kinds = [e["ev"] for e in events]
+    assert "error" in kinds and kinds[-1] == "done" and events[-1]["code"] == 5 and events[-1]["partial"]
