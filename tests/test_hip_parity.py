@@ -131,3 +131,32 @@ def test_every_launch_shape_is_bitwise_the_oracle(faces_threads, cells_threads):
     ora.advance(5.0)
     assert hip.stepper.nsteps == ora.stepper.nsteps
     _compare(hip, ora, 0.0, f"launch {faces_threads}/{cells_threads}")
+
+
+@pytest.mark.gpu
+def test_a_momentum_or_depth_set_between_steps_is_taken_as_given():
+    """The device keeps no face momentum between steps (the face kernel recomputes it from the velocity and the
+    depths the last step read): reading it, then setting it (a kick, as FaceKick.apply writes b.mx[faces],
+    momentum.py:49-54) and setting the depth between steps must leave every field bit for bit the oracle's, which
+    stores the momentum as the reference does."""
+    tables, h0 = rough_dam_break()
+    num = Numerics(dt_every=1, dt_max=0.2)
+    hip, ora = _pair(tables, num, "hip-exact", 4, h0=h0)
+    for m in (hip, ora):
+        m.advance(5.0)
+    _compare(hip, ora, 0.0, "exact f32 t=5 (momentum stored on read)")
+    from wadi2d_amd.solver.state import to_state
+
+    for m in (hip, ora):
+        mx = m.field("mx").astype(np.float32)
+        mx[:, ::7] *= np.float32(1.5)  # a kick on every seventh column of faces
+        m.backend.set_field("mx", to_state(m.grid, mx, np.float32))
+    for m in (hip, ora):
+        m.advance(8.0)
+    _compare(hip, ora, 0.0, "exact f32 t=8 after a kick")
+    for m in (hip, ora):
+        h = m.field("h").astype(np.float32)
+        h[::11, :] += np.where(m.grid.mask[::11, :], np.float32(0.25), np.float32(0.0))
+        m.backend.set_field("h", to_state(m.grid, h, np.float32))
+        m.advance(12.0)
+    _compare(hip, ora, 0.0, "exact f32 t=12 after a depth change")
