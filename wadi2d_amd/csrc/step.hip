@@ -1286,19 +1286,31 @@ struct RunLists {
 // their neighbours (retain); 2 does that and lets a tile go after SLEEP_CHECKS checks that did not want it (check);
 // 3 drops the member's tiles (done). The list stays in ascending order; woken and leaving tiles go to their lists.
 // One workgroup, the member's own tiles.
-__device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[PRE_THREADS / 64], int& n_woken,
-                        int& n_leaving) {
+constexpr int SHARED_TILES = 32768;  // a member with no more tiles keeps its tile flags in LDS, one byte each
+
+// SH: the flags in LDS (`sf`, indexed from tile0): bit 0 wanted, bit 1 active after, bit 2 active before; else in
+// L.flags (bits 0 and 1), the before flag read again from is_active. Loops over the member's tiles are unrolled so
+// that several tiles' loads are in flight at once.
+template <bool SH>
+__device__ void rebuild_flags(Member& b, int mode, const RunLists& L, int (*wave_cnt)[PRE_THREADS / 64], int& n_woken,
+                              int& n_leaving, unsigned char* __restrict__ sf) {
     const int t0 = b.tile0, t1 = b.tile1;
     const bool look = mode == 1 || mode == 2;
+    const bool raining = b.scale != 0.0;
+#pragma unroll 4
     for (int t = t0 + threadIdx.x; t < t1; t += PRE_THREADS) {
         int wanted = 0;
         if (look) {
-            const int keep = (L.has_source[t] && b.scale != 0.0) || L.bnd_tile[t];
+            const int keep = (L.has_source[t] && raining) || L.bnd_tile[t];
             wanted = L.wet[t] || keep;
         }
-        L.flags[t] = wanted;
+        if constexpr (SH)
+            sf[t - t0] = (unsigned char)wanted;
+        else
+            L.flags[t] = wanted;
     }
     __syncthreads();
+#pragma unroll 2
     for (int t = t0 + threadIdx.x; t < t1; t += PRE_THREADS) {
         const int cur = L.is_active[t];
         int a = cur;
@@ -1306,8 +1318,19 @@ __device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[
             a = 0;
         } else if (look) {
             // neighbours are this member's tiles or the void tile, whose flag is never set
-            a = L.flags[t] & 1;
-            for (int q = 0; q < 8; ++q) a |= L.flags[L.nbr[(size_t)t * 8 + q]] & 1;
+            int nb[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) nb[q] = L.nbr[(size_t)t * 8 + q];
+            if constexpr (SH) {
+                a = sf[t - t0] & 1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q)
+                    a |= (unsigned)(nb[q] - t0) < (unsigned)(t1 - t0) ? sf[nb[q] - t0] & 1 : 0;
+            } else {
+                a = L.flags[t] & 1;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) a |= L.flags[nb[q]] & 1;
+            }
             if (mode == 1) {
                 a |= cur;
             } else {
@@ -1315,15 +1338,25 @@ __device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[
                 a |= cur && L.idle[t] < SLEEP_CHECKS;
             }
         }
-        L.flags[t] |= a << 1;
+        if constexpr (SH)
+            sf[t - t0] |= (unsigned char)((a << 1) | (cur << 2));
+        else
+            L.flags[t] |= a << 1;
     }
     __syncthreads();
     int3 n = make_int3(0, 0, 0);  // active, woken, leaving
     for (int base = t0; base < t1; base += PRE_THREADS) {
         const int t = base + threadIdx.x;
         const bool in = t < t1;
-        const int a = in ? (L.flags[t] >> 1) & 1 : 0;
-        const int cur = in ? L.is_active[t] : 0;
+        int a, cur;
+        if constexpr (SH) {
+            const int f = in ? sf[t - t0] : 0;
+            a = (f >> 1) & 1;
+            cur = (f >> 2) & 1;
+        } else {
+            a = in ? (L.flags[t] >> 1) & 1 : 0;
+            cur = in ? L.is_active[t] : 0;
+        }
         const bool wk = a && !cur && look;
         const bool lv = cur && !a;
         const int3 slot = chunk_slots(a, wk, lv, wave_cnt, n);
@@ -1337,6 +1370,14 @@ __device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[
     n_woken = n.y;
     n_leaving = n.z;
     __syncthreads();
+}
+
+__device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[PRE_THREADS / 64], int& n_woken,
+                        int& n_leaving, unsigned char* sf) {
+    if (b.tile1 - b.tile0 <= SHARED_TILES)
+        rebuild_flags<true>(b, mode, L, wave_cnt, n_woken, n_leaving, sf);
+    else
+        rebuild_flags<false>(b, mode, L, wave_cnt, n_woken, n_leaving, sf);
 }
 
 // What a change of the list does to tiles (stepper.py:94-99): a woken tile's wet share from its depth, by the
@@ -1409,11 +1450,12 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
     static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
     __shared__ int wave_cnt[3][PRE_THREADS / 64];
     __shared__ int any;
+    __shared__ unsigned char sflags[SHARED_TILES];
     Member& b = mem[blockIdx.x];
     int nw, nl;
     bool changed = false;
     if (b.check) {
-        rebuild(b, 2, L, wave_cnt, nw, nl);
+        rebuild(b, 2, L, wave_cnt, nw, nl, sflags);
         transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last, dx, fresh);
         if (threadIdx.x == 0) b.check = 0;
         changed = true;
@@ -1427,7 +1469,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         }
         __syncthreads();
         if (any) {
-            rebuild(b, b.rebuild, L, wave_cnt, nw, nl);
+            rebuild(b, b.rebuild, L, wave_cnt, nw, nl, sflags);
             transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last, dx, fresh);
             changed = true;
         }
