@@ -401,3 +401,35 @@ def test_every_boundary_kind_and_storm_kind_runs_and_a_collapsing_step_ends_in_e
     events = [json.loads(x) for x in (job_folder / "fail" / "progress.jsonl").read_text().splitlines()]
     kinds = [e["ev"] for e in events]
     assert "error" in kinds and kinds[-1] == "done" and events[-1]["code"] == 5 and events[-1]["partial"]
+
+
+# ---- the run folder's files
+
+
+def test_progress_results_and_checkpoint_files_round_trip(tmp_path):
+    from wadi2d_amd.io import Progress, Results, load_checkpoint, save_checkpoint, write_series_csv
+
+    with Progress(tmp_path / "progress.jsonl") as p:
+        p.emit("start", cells=3, dx=10.0)
+        p.emit("done", code=0, balance={"rain_m3": 1.5})
+    lines = [json.loads(x) for x in (tmp_path / "progress.jsonl").read_text().splitlines()]
+    assert lines == [{"ev": "start", "cells": 3, "dx": 10.0}, {"ev": "done", "code": 0, "balance": {"rain_m3": 1.5}}]
+    g = _grid(8, 8)
+    r = Results(tmp_path / "results", g, "{}")
+    r.snapshot(900.4, {"depth": np.ones(g.shape)})
+    r.maxima({"depth": np.full(g.shape, 2.0)})
+    r.series("lines", "a b", np.array([0.0, 60.0]), {"discharge_m3s": [1.0, 2.0]})
+    r.attributes({"status": "finished"})
+    assert np.load(tmp_path / "results" / "maps" / "t000000900" / "depth.npy").dtype == np.float32
+    assert json.loads((tmp_path / "results" / "attributes.json").read_text())["status"] == "finished"
+    with np.load(tmp_path / "results" / "series" / "lines" / "a_b.npz") as f:
+        assert list(f["discharge_m3s"]) == [1.0, 2.0]
+    again = Results(tmp_path / "results", g, "{}", append=True)
+    assert again.attrs["status"] == "finished"
+    write_series_csv(tmp_path / "s.csv", [0.0, 60.0], {"q": [1.0, 2.5]})
+    assert (tmp_path / "s.csv").read_text().splitlines() == ["t_s,q", "0.000,1", "60.000,2.5"]
+    save_checkpoint(tmp_path / "checkpoint.npz", {"job_hash": "abc", "t": 30.0, "h": np.arange(4.0)})
+    back = load_checkpoint(tmp_path / "checkpoint.npz")
+    assert back["job_hash"] == "abc" and back["t"] == 30.0 and np.array_equal(back["h"], np.arange(4.0))
+    (tmp_path / "bad.npz").write_bytes(b"not a checkpoint")
+    assert load_checkpoint(tmp_path / "bad.npz") is None
