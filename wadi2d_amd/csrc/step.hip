@@ -1388,31 +1388,66 @@ __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, 
                             R* __restrict__ h_other, R* __restrict__ om, R* const* __restrict__ faces_cur,
                             R* const* __restrict__ faces_other, const float* __restrict__ hc,
                             const float2* __restrict__ ctab, size_t ncell, int last, R dx, int fresh) {
-    for (int q = 0; q < nw; ++q) {
-        const size_t c = (size_t)L.woken[b.tile0 + q] * TILE2 + threadIdx.x;
-        om[c] = cell_plan<R, FAST>(h[c], R(hc[c]), ctab + c, ncell, last);
+    // tiles are taken TB at a time, every load of a batch issued before its stores (a member that finishes moves all
+    // its tiles here at once)
+    constexpr int TB = 4;
+    const int p = threadIdx.x, j = p / TILE, i = p - (p / TILE) * TILE;
+    for (int q0 = 0; q0 < nw; q0 += TB) {
+        R hv[TB], hcv[TB];
+        size_t cv[TB];
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            cv[u] = q0 + u < nw ? (size_t)L.woken[b.tile0 + q0 + u] * TILE2 + p : 0;
+            hv[u] = q0 + u < nw ? h[cv[u]] : R(0);
+            hcv[u] = q0 + u < nw ? R(hc[cv[u]]) : R(0);
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u)
+            if (q0 + u < nw) om[cv[u]] = cell_plan<R, FAST>(hv[u], hcv[u], ctab + cv[u], ncell, last);
     }
     // a leaving tile's face momentum is stored (both buffers) from the depths the last step read, before those
-    // depth buffers are made one; a stored momentum (set from the host since) is already there
+    // depth buffers are made one; a stored momentum (set from the host since) is already in both
     if (nl && !fresh) {
-        for (int q = 0; q < nl; ++q) {
-            const int t = L.leaving[b.tile0 + q];
-            const int p = threadIdx.x, j = p / TILE, i = p - (p / TILE) * TILE;
-            const size_t c = (size_t)t * TILE2 + p;
-            const size_t w = i > 0 ? c - 1 : (size_t)L.nbr[(size_t)t * 8 + 3] * TILE2 + j * TILE + (TILE - 1);
-            const size_t n = j > 0 ? c - TILE : (size_t)L.nbr[(size_t)t * 8 + 1] * TILE2 + (TILE - 1) * TILE + i;
-            const R a = R(0.5) * (h_other[w] + h_other[c]) * dx * faces_cur[0][c];
-            const R e = R(0.5) * (h_other[n] + h_other[c]) * dx * faces_cur[1][c];
-            faces_cur[2][c] = a, faces_other[2][c] = a;
-            faces_cur[3][c] = e, faces_other[3][c] = e;
+        for (int q0 = 0; q0 < nl; q0 += TB) {
+            R av[TB], ev[TB];
+            size_t cv[TB];
+#pragma unroll
+            for (int u = 0; u < TB; ++u) {
+                if (q0 + u >= nl) continue;
+                const int t = L.leaving[b.tile0 + q0 + u];
+                const size_t c = (size_t)t * TILE2 + p;
+                const size_t w = i > 0 ? c - 1 : (size_t)L.nbr[(size_t)t * 8 + 3] * TILE2 + j * TILE + (TILE - 1);
+                const size_t n = j > 0 ? c - TILE : (size_t)L.nbr[(size_t)t * 8 + 1] * TILE2 + (TILE - 1) * TILE + i;
+                cv[u] = c;
+                av[u] = R(0.5) * (h_other[w] + h_other[c]) * dx * faces_cur[0][c];
+                ev[u] = R(0.5) * (h_other[n] + h_other[c]) * dx * faces_cur[1][c];
+            }
+#pragma unroll
+            for (int u = 0; u < TB; ++u) {
+                if (q0 + u >= nl) continue;
+                faces_cur[2][cv[u]] = av[u], faces_other[2][cv[u]] = av[u];
+                faces_cur[3][cv[u]] = ev[u], faces_other[3][cv[u]] = ev[u];
+            }
         }
         __syncthreads();
     }
-    for (int q = 0; q < nl; ++q) {
-        const size_t c = (size_t)L.leaving[b.tile0 + q] * TILE2 + threadIdx.x;
-        h_other[c] = h[c];
-        if (b.done)
-            for (int f = 0; f < 4; ++f) faces_other[f][c] = faces_cur[f][c];
+    // then one mean depth in both buffers, and for a finished member one velocity (the momentum is in both already)
+    for (int q0 = 0; q0 < nl; q0 += TB) {
+        R hv[TB], uv[TB], vv[TB];
+        size_t cv[TB];
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            if (q0 + u >= nl) continue;
+            cv[u] = (size_t)L.leaving[b.tile0 + q0 + u] * TILE2 + p;
+            hv[u] = h[cv[u]];
+            if (b.done) uv[u] = faces_cur[0][cv[u]], vv[u] = faces_cur[1][cv[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < TB; ++u) {
+            if (q0 + u >= nl) continue;
+            h_other[cv[u]] = hv[u];
+            if (b.done) faces_other[0][cv[u]] = uv[u], faces_other[1][cv[u]] = vv[u];
+        }
     }
     __syncthreads();
 }
