@@ -60,7 +60,11 @@ int w2d_create(const w2d_tables* tables, const w2d_params* params, int32_t preci
 int w2d_destroy(w2d_handle* h);
 
 /* set_field / get_field (gpu/backend.py:121-125): n = ntiles * 1296 entries of the handle's type. get_field fills
- * every halo entry from the tile that owns the cell, as the reference's exchange leaves them. */
+ * every halo entry from the tile that owns the cell, as the reference's exchange leaves them. The face momentum
+ * (MX, MY) is not stored by a step: the face kernel recomputes the last step's 0.5 (hL + hR) dx u from the velocity
+ * and the depths that step read, bit for bit the product it replaces; get_field and read_cells of MX, MY store it
+ * first, and a set_field of H, U, V, MX or MY stores it before the change, so the next step takes the momentum as
+ * the reference would hold it. MX1 and MY1 read and write MX and MY (one momentum, both buffers). */
 int w2d_set_field(w2d_handle* h, int32_t field, const void* host, int64_t n);
 int w2d_get_field(w2d_handle* h, int32_t field, void* host, int64_t n);
 /* Model.read_cells (model.py:139-143): entries at own-cell state indices. */

@@ -433,3 +433,23 @@ def test_progress_results_and_checkpoint_files_round_trip(tmp_path):
     assert back["job_hash"] == "abc" and back["t"] == 30.0 and np.array_equal(back["h"], np.arange(4.0))
     (tmp_path / "bad.npz").write_bytes(b"not a checkpoint")
     assert load_checkpoint(tmp_path / "bad.npz") is None
+
+
+@pytest.mark.gpu
+def test_the_command_line_runs_a_job_on_the_device(job_folder):
+    """``python -m wadi2d_amd run job.json --out DIR --backend hip-f64``: exit 0, the events on stdout as in the file,
+    the maxima and the series written; a second run with --resume and no stop finds the finished run's checkpoint
+    missing (none is written on finishing) and starts again with a warning."""
+    d = job_dict(duration=120.0)
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    cmd = [sys.executable, "-m", "wadi2d_amd", "run", str(job_folder / "job.json"), "--out", str(job_folder / "o"),
+           "--backend", "hip-f64"]  # fmt: skip
+    r = subprocess.run(cmd, capture_output=True, text=True, cwd=ROOT, timeout=240)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0
+    out = [json.loads(x) for x in r.stdout.splitlines() if x.startswith("{")]
+    file = [json.loads(x) for x in (job_folder / "o" / "progress.jsonl").read_text().splitlines()]
+    assert out == file and file[-1]["ev"] == "done" and file[-1]["code"] == 0
+    assert np.load(job_folder / "o" / "results" / "maxima" / "depth.npy").max() > 0.1
+    r = subprocess.run([*cmd, "--resume"], capture_output=True, text=True, cwd=ROOT, timeout=240)
+    assert r.returncode == 0 and "no checkpoint to resume from" in r.stdout
