@@ -83,7 +83,8 @@ class Driver:
         self.balance.rain += rain * share
         self.settled = min(t, t1)
 
-    def _on_step(self, st) -> None:
+    def on_step(self, st) -> None:
+        """After every step: the boundary tally, the maxima every ``maxima_every`` steps, the step budget."""
         self.b.tally_add(st.t - self.last_t)
         self.last_t = st.t
         if self.peak_flow:
@@ -93,7 +94,8 @@ class Driver:
         if self.budget is not None and st.nsteps >= self.budget:
             raise StepBudgetSpent
 
-    def _open(self, t: float, t_next: float) -> None:
+    def open_window(self, t: float, t_next: float) -> None:
+        """The source of window [t, t_next]: its rate on the device, its tiles kept, a forced refresh, its rain."""
         s = self.rain.scale(t, t_next)
         if s != 0.0 or not self.dry:
             self.b.source_scale(s)
@@ -119,8 +121,8 @@ class Driver:
             while st.t < t_end - 1e-9:
                 if st.t >= self.window[1] - 1e-9 or self.window == (0.0, 1.0, 0.0):
                     t_next = min((np.floor(st.t / self.every + 1e-9) + 1.0) * self.every, t_end)
-                    self._open(st.t, t_next)
-                self.model.advance(self.window[1], self._on_step)
+                    self.open_window(st.t, t_next)
+                self.model.advance(self.window[1], self.on_step)
                 self.settle(st.t)
         except StepBudgetSpent:
             self.settle(st.t)

@@ -23,7 +23,6 @@ from pathlib import Path
 
 import numpy as np
 
-from wadi2d_amd.config import MM_PER_M
 from wadi2d_amd.dem import grid_over, read_raster
 from wadi2d_amd.engine import forcing
 from wadi2d_amd.engine.driver import Balance, Driver
@@ -224,7 +223,7 @@ class Monitor:
             self.stop_due = True
 
     def on_step(self, st) -> None:
-        self.drv._on_step(st)
+        self.drv.on_step(st)
         t = st.t
         if st.dt < self.dt_min:
             self.dt_min = st.dt
@@ -254,7 +253,7 @@ class Monitor:
 
     # checkpoint (io/checkpoint.py)
     def save(self, path: Path, key: str, t: float) -> None:
-        m, b, st = self.model, self.model.backend, self.model.stepper
+        b, st = self.model.backend, self.model.stepper
         b.synchronize()
         payload = {
             "job_hash": key, "t": t, "t_stepper": st.t, "dt": st.dt, "nsteps": st.nsteps, "refresh_due": st.refresh_due,
@@ -273,7 +272,6 @@ class Monitor:
                 for q, values in cols.items():
                     payload[f"series_{kind}_{n}_{q}"] = np.asarray(values, np.float64)
         save_checkpoint(path, payload)
-        del m
 
     def load(self, saved: dict) -> float:
         b, st = self.model.backend, self.model.stepper
@@ -356,7 +354,7 @@ def _run(cfg: RunConfig, out: Path, backend: str | None, resume: bool, progress:
             raise StopRequested
         while t < end - 1e-9:
             t_next = min(t + cfg.run.source_every_s, end)
-            drv._open(t, t_next)
+            drv.open_window(t, t_next)
             model.advance(t_next, monitor.on_step)
             drv.settle(t_next)
             t = t_next
@@ -442,4 +440,4 @@ def run_job(cfg: RunConfig, out: str | Path, backend: str | None = None, resume:
     return summary
 
 
-__all__ = ["RunSummary", "run_job", "storm_curve", "MM_PER_M"]
+__all__ = ["RunSummary", "run_job", "storm_curve"]
