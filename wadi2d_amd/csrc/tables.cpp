@@ -10,6 +10,8 @@
 //   driver, inverts ....... wadi2d/tables/builder.py:131-258, 261-269, 290-308  (build_tables, tile_margin, _faces)
 // Each tile writes only its own slots, so the tables do not depend on the thread count (builder.py:147-148).
 
+#include "table_rows.hpp"
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -19,10 +21,9 @@
 
 namespace {
 
-constexpr int TILE = 32;
-constexpr double EPS_RELIEF = 1e-9;          // core.py:13
-constexpr double CLOSED = 1.0e9;             // solver/layout.py:41
-constexpr double CRITICAL_WEIR = 1.7046;     // config/constants.py:6, (2/3) sqrt(2 g / 3)
+using w2d_rows::CLOSED;
+using w2d_rows::CRITICAL_WEIR;
+using w2d_rows::TILE;
 
 struct Bed {  // builder.py:38-56 ArrayBed: reads clip to the edge pixel
     const double* z;
@@ -38,73 +39,24 @@ struct Bed {  // builder.py:38-56 ArrayBed: reads clip to the edge pixel
 void cell_table(std::vector<double>& p, int rows, double* tab, double& zmin, double& relmean, double& hc,
                 std::vector<double>& hbar_at) {
     std::sort(p.begin(), p.end());
-    const int n = (int)p.size();
-    zmin = p[0];
-    for (int q = 0; q < n; ++q) p[q] -= zmin;  // rel
-    double sum = 0.0;
-    for (int q = 0; q < n; ++q) sum += p[q];
-    relmean = sum / n;
-    hc = p[n - 1] - relmean;
-    if (hc <= EPS_RELIEF) {
-        for (int r = 0; r < rows; ++r) tab[r] = 0.0;
-        hc = 0.0;
-        return;
-    }
-    double cum = 0.0;
-    hbar_at.resize(n);
-    for (int m = 0; m < n; ++m) {
-        cum += p[m];
-        hbar_at[m] = (p[m] * (m + 1) - cum) / n;
-    }
-    const int last = rows - 1;
-    int m = 0;
-    for (int r = 0; r < rows; ++r) {
-        const double x = (double)r / last;
-        const double hb = hc * (x * x);
-        while (m + 1 < n && hbar_at[m + 1] <= hb) ++m;
-        tab[r] = p[m] + (hb - hbar_at[m]) * n / (m + 1);
-    }
-    tab[last] = p[n - 1];
+    hbar_at.resize(p.size());
+    w2d_rows::cell_rows(p.data(), (int)p.size(), rows, tab, zmin, relmean, hc, hbar_at.data());
 }
 
 // core.py:44-79. prof is modified (crest). tab is (rows, 2). Returns invert zf and table top.
-void face_table(std::vector<double>& prof, double crest, double pixel, double n_inv, const double* weight,
-                double length, double e_lo, double e_hi, int rows, double* tab, double& zf, double& top) {
-    const int m = (int)prof.size();
-    if (!std::isnan(crest))
-        for (int q = 0; q < m; ++q) prof[q] = std::max(prof[q], crest);
-    zf = prof[0];
-    for (int q = 1; q < m; ++q) zf = std::min(zf, prof[q]);
-    double relmax = -std::numeric_limits<double>::infinity();
-    for (int q = 0; q < m; ++q) relmax = std::max(relmax, prof[q] - zf);
-    top = std::max(std::min(relmax, e_hi), e_lo);
-    const int last = rows - 1;
-    for (int r = 0; r < rows; ++r) {
-        const double x = (double)r / last;
-        const double e = top * (x * x);
-        double area = 0.0, conv = 0.0;
-        for (int q = 0; q < m; ++q) {
-            const double dep = e - (prof[q] - zf);
-            if (dep > 0.0) {
-                area += dep;
-                const double c = std::cbrt(dep);
-                conv += weight[q] * dep * (c * c);
-            }
-        }
-        tab[2 * r] = pixel * area * length;
-        tab[2 * r + 1] = std::pow(pixel * n_inv * conv * length, 0.6);
-    }
+void face_table(std::vector<double>& prof, double crest, double pixel, double n_inv, double length, double e_lo,
+                double e_hi, int rows, double* tab, double& zf, double& top) {
+    w2d_rows::face_rows(prof.data(), (int)prof.size(), crest, pixel, n_inv, length, e_lo, e_hi, rows, tab, zf, top);
 }
 
 // core.py:82-179: overwrite the conveyance column with the local hydraulic radius.
 struct RadiusScratch {
-    std::vector<double> d, per, s0, s1, p0, p1;
-    std::vector<int64_t> run_lo, run_hi;
+    std::vector<double> d, per, lo, hi, s0, s1, p0, p1;
     void size(int n) {
         d.assign(n, 0.0);
         per.assign(n, 0.0);
-        run_lo.assign(n, 0);
-        run_hi.assign(n, 0);
+        lo.assign(n, 0.0);
+        hi.assign(n, 0.0);
         s0.assign(n + 1, 0.0);
         s1.assign(n + 1, 0.0);
         p0.assign(n + 1, 0.0);
@@ -113,86 +65,15 @@ struct RadiusScratch {
 };
 
 void local_radius_conveyance(const double* ext, int n, int i0, int i1, double zf, double top, double pixel,
-                             double n_inv, const double* weight, double length, double beta, int rows, double* tab,
-                             RadiusScratch& S) {
+                             double n_inv, double length, double beta, int rows, double* tab, RadiusScratch& S) {
     S.size(n);
-    const int last = rows - 1;
-    const double pix2 = pixel * pixel;
-    for (int r = 0; r < rows; ++r) {
-        const double x = (double)r / last;
-        const double level = zf + top * (x * x);
-        for (int t = 0; t < n; ++t) {
-            const double dep = level - ext[t];
-            S.d[t] = dep > 0.0 ? dep : 0.0;
-        }
-        for (int t = 0; t < n; ++t) S.per[t] = 0.0;
-        if (S.d[0] > 0.0) S.per[0] += 0.5 * pixel;
-        if (S.d[n - 1] > 0.0) S.per[n - 1] += 0.5 * pixel;
-        for (int t = 0; t < n - 1; ++t) {
-            const bool wa = S.d[t] > 0.0, wb = S.d[t + 1] > 0.0;
-            if (!(wa || wb)) continue;
-            const double rise = ext[t + 1] - ext[t];
-            const double seg = std::sqrt(pix2 + rise * rise);
-            if (wa && wb) {
-                S.per[t] += 0.5 * seg;
-                S.per[t + 1] += 0.5 * seg;
-            } else if (wa) {
-                S.per[t] += rise > 0.0 ? std::min(S.d[t] / rise, 1.0) * seg : 0.5 * pixel;
-            } else if (-rise > 0.0) {
-                S.per[t + 1] += std::min(S.d[t + 1] / -rise, 1.0) * seg;
-            } else {
-                S.per[t + 1] += 0.5 * pixel;
-            }
-        }
-        int64_t start = -1;
-        for (int t = 0; t < n; ++t) {
-            if (S.d[t] > 0.0) {
-                if (start < 0) start = t;
-                S.run_lo[t] = start;
-            } else {
-                start = -1;
-            }
-        }
-        int64_t end = -1;
-        for (int t = n - 1; t >= 0; --t) {
-            if (S.d[t] > 0.0) {
-                if (end < 0) end = t;
-                S.run_hi[t] = end;
-            } else {
-                end = -1;
-            }
-        }
-        for (int t = 0; t < n; ++t) {
-            S.s0[t + 1] = S.s0[t] + S.d[t];
-            S.s1[t + 1] = S.s1[t] + t * S.d[t];
-            S.p0[t + 1] = S.p0[t] + S.per[t];
-            S.p1[t + 1] = S.p1[t] + t * S.per[t];
-        }
-        double conv = 0.0;
-        for (int i = i0; i < i1; ++i) {
-            const double di = S.d[i];
-            if (di <= 0.0) continue;
-            const double w = std::max(beta * di / pixel, 1e-9);
-            const int64_t lo = std::max(S.run_lo[i], (int64_t)std::ceil(i - w));
-            const int64_t hi = std::min(S.run_hi[i], (int64_t)std::floor(i + w));
-            double a_sum = (1.0 - i / w) * (S.s0[i + 1] - S.s0[lo]) + (S.s1[i + 1] - S.s1[lo]) / w;
-            a_sum += (1.0 + i / w) * (S.s0[hi + 1] - S.s0[i + 1]) - (S.s1[hi + 1] - S.s1[i + 1]) / w;
-            double p_sum = (1.0 - i / w) * (S.p0[i + 1] - S.p0[lo]) + (S.p1[i + 1] - S.p1[lo]) / w;
-            p_sum += (1.0 + i / w) * (S.p0[hi + 1] - S.p0[i + 1]) - (S.p1[hi + 1] - S.p1[i + 1]) / w;
-            const double radius = p_sum > 0.0 ? pixel * a_sum / p_sum : di;
-            const double c = std::cbrt(radius);
-            conv += weight[i - i0] * di * (c * c);
-        }
-        tab[2 * r + 1] = std::pow(pixel * n_inv * conv * length, 0.6);
-    }
+    w2d_rows::local_radius_rows(const_cast<double*>(ext), n, i0, i1, zf, top, pixel, n_inv, length, beta, rows, tab,
+                                S.d.data(), S.per.data(), S.lo.data(), S.hi.data(), S.s0.data(), S.s1.data(),
+                                S.p0.data(), S.p1.data());
 }
 
-// core.py:354-362
 double window_top(const double* line, int n, double zf, double top, double e_hi) {
-    double hi = -std::numeric_limits<double>::infinity();
-    for (int t = 0; t < n; ++t)
-        if (line[t] > hi) hi = line[t];
-    return std::max(top, std::min(hi - zf, e_hi));
+    return w2d_rows::window_top(const_cast<double*>(line), n, zf, top, e_hi);
 }
 
 struct Config {
@@ -265,7 +146,7 @@ int w2d_build_tables(const double* bed, int64_t bed_ny, int64_t bed_nx, int64_t 
         const int W = TILE * k + 1;
         const int WB = W + 2 * margin;
         std::vector<double> zbig((size_t)WB * WB);
-        std::vector<double> pix((size_t)k * k), hbar_at, prof(k), weight(k, 1.0), full(k + 2 * margin);
+        std::vector<double> pix((size_t)k * k), hbar_at, prof(k), full(k + 2 * margin);
         std::vector<double> ctab(rows), ftx((size_t)rows * 2), fty((size_t)rows * 2);
         RadiusScratch S;
 #pragma omp for schedule(dynamic, 1)
@@ -297,13 +178,13 @@ int w2d_build_tables(const double* bed, int64_t bed_ny, int64_t bed_nx, int64_t 
                     for (int q = 0; q < k; ++q) prof[q] = std::max(z(r0 + q, c0 - 1), z(r0 + q, c0));
                     double zfx, efx;
                     const double xz = crest_at(crest_xz, gj, gi, NaN);
-                    face_table(prof, xz, pixel, n_inv, weight.data(), crest_at(crest_xlen, gj, gi, 1.0), cfg->e_lo,
+                    face_table(prof, xz, pixel, n_inv, crest_at(crest_xlen, gj, gi, 1.0), cfg->e_lo,
                                cfg->e_hi, rows, ftx.data(), zfx, efx);
                     // y face (north) (core.py:257-264)
                     for (int q = 0; q < k; ++q) prof[q] = std::max(z(r0 - 1, c0 + q), z(r0, c0 + q));
                     double zfy, efy;
                     const double yz = crest_at(crest_yz, gj, gi, NaN);
-                    face_table(prof, yz, pixel, n_inv, weight.data(), crest_at(crest_ylen, gj, gi, 1.0), cfg->e_lo,
+                    face_table(prof, yz, pixel, n_inv, crest_at(crest_ylen, gj, gi, 1.0), cfg->e_lo,
                                cfg->e_hi, rows, fty.data(), zfy, efy);
                     if (cfg->local_radius) {
                         // core.py:332-351; crest faces keep their table
@@ -316,12 +197,12 @@ int w2d_build_tables(const double* bed, int64_t bed_ny, int64_t bed_nx, int64_t 
                             const double top = window_top(full.data(), nfull, zfx, efx, cfg->e_hi);
                             if (top > efx) {  // _raise_top, core.py:365-371
                                 for (int q = 0; q < k; ++q) prof[q] = full[margin + q];
-                                face_table(prof, NaN, pixel, n_inv, weight.data(), len, top, cfg->e_hi, rows,
+                                face_table(prof, NaN, pixel, n_inv, len, top, cfg->e_hi, rows,
                                            ftx.data(), zfx, efx);
                             }
                             const int e = std::min(margin, (int)std::ceil(cfg->beta * efx / pixel));
                             local_radius_conveyance(full.data() + margin - e, k + 2 * e, e, e + k, zfx, efx, pixel,
-                                                    n_inv, weight.data(), len, cfg->beta, rows, ftx.data(), S);
+                                                    n_inv, len, cfg->beta, rows, ftx.data(), S);
                         }
                         if (std::isnan(yz)) {
                             const double len = crest_at(crest_ylen, gj, gi, 1.0);
@@ -331,12 +212,12 @@ int w2d_build_tables(const double* bed, int64_t bed_ny, int64_t bed_nx, int64_t 
                             const double top = window_top(full.data(), nfull, zfy, efy, cfg->e_hi);
                             if (top > efy) {
                                 for (int q = 0; q < k; ++q) prof[q] = full[margin + q];
-                                face_table(prof, NaN, pixel, n_inv, weight.data(), len, top, cfg->e_hi, rows,
+                                face_table(prof, NaN, pixel, n_inv, len, top, cfg->e_hi, rows,
                                            fty.data(), zfy, efy);
                             }
                             const int e = std::min(margin, (int)std::ceil(cfg->beta * efy / pixel));
                             local_radius_conveyance(full.data() + margin - e, k + 2 * e, e, e + k, zfy, efy, pixel,
-                                                    n_inv, weight.data(), len, cfg->beta, rows, fty.data(), S);
+                                                    n_inv, len, cfg->beta, rows, fty.data(), S);
                         }
                     }
                     zfx_g[g] = zfx;
