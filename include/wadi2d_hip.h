@@ -15,6 +15,8 @@
 
 #include <stdint.h>
 
+#include "wadi2d_host.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -205,6 +207,19 @@ int w2d_mlp_adam(float* params, float* m, float* v, const float* grad, int32_t s
                  float beta2, float eps, uint16_t* wbf16, void* stream);
 /* Devices the HIP runtime sees (select_backend's probe, solver/backend.py:58-75). */
 int w2d_device_count(int32_t* out);
+
+/* build_tables (tables/builder.py:131-258) on the device: the same arguments, results and statements as
+ * w2d_build_tables (wadi2d_host.h), one device thread per cell in float64, the host's arrays copied in and the
+ * results copied back before it returns. The reference builds its tables on the CPU only (builder.py:147-148
+ * spreads tiles over processes); this is the faster builder for large grids. The device's cbrt and pow may differ
+ * from the host library's in the last bit, so a row may differ from the host builder's by one float32 ulp.
+ * Returns 0, 1 on a bad argument, 2 when the device is out of memory, 3 on another HIP error. */
+int w2d_build_tables_device(const double* bed, int64_t bed_ny, int64_t bed_nx, int64_t row_off, int64_t col_off,
+                            double pixel, int32_t ny, int32_t nx, double dx, const int32_t* tiles, int32_t ntiles,
+                            const uint8_t* mask, const double* manning, int32_t nman_grid, const double* crest_xz,
+                            const double* crest_xlen, const double* crest_xc, const double* crest_yz,
+                            const double* crest_ylen, const double* crest_yc, const w2d_table_config* cfg,
+                            w2d_table_out* out);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ ROOT = HERE.parent
 ARCH = os.environ.get("WADI2D_OFFLOAD_ARCH", "gfx950")
 
 HOST_SOURCES = ["tables.cpp", "synth.cpp"]
-HIP_SOURCES = ["step.hip", "mlp.hip"]
+HIP_SOURCES = ["step.hip", "mlp.hip", "tables_gpu.hip"]
 
 
 def _run(cmd: list[str]) -> None:

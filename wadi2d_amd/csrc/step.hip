@@ -2031,7 +2031,7 @@ int big_sum(w2d_handle* H, int64_t n, const int64_t* index, const R* a, const R*
 
 }  // namespace
 
-// Shared with the library's other translation unit (csrc/mlp.hip): record the message w2d_last_error returns, and
+// Shared with the library's other translation units (csrc/mlp.hip, csrc/tables_gpu.hip): record the message w2d_last_error returns, and
 // return `code`.
 int w2d_report(int code, const char* msg) { return fail(code, msg); }
 

@@ -6,8 +6,9 @@
 //   local radius .......... wadi2d/tables/core.py:82-179  (_local_radius_conveyance, Spada et al. 2015 eqs. 14-17)
 //   window top ............ wadi2d/tables/core.py:354-362 (window_top)
 //
-// Scratch arrays are reached through an accessor `A` (a[q] -> double&): a plain pointer on the host, a view whose
-// element q of thread i lies at base[q * stride + i] on the device (neighbouring threads read neighbouring words).
+// Scratch arrays and the rows written are reached through accessors (a[q] -> double&): a plain pointer on the host,
+// a view whose element q of thread i lies at base[q * stride + i] on the device (neighbouring threads read
+// neighbouring words).
 // min and max are std::min/std::max's comparisons, so NaN and signed zeros go as on the host.
 
 #pragma once
@@ -39,8 +40,8 @@ struct Strided {
 };
 
 // core.py:16-41 on sorted pixels p[0..n): zmin, mean bed over zmin, Hc; tab[rows]; hb is scratch of n.
-template <class A>
-W2D_HD void cell_rows(A p, int n, int rows, double* tab, double& zmin, double& relmean, double& hc, A hb) {
+template <class A, class T>
+W2D_HD void cell_rows(A p, int n, int rows, T tab, double& zmin, double& relmean, double& hc, A hb) {
     zmin = p[0];
     for (int q = 0; q < n; ++q) p[q] -= zmin;  // rel
     double sum = 0.0;
@@ -69,9 +70,9 @@ W2D_HD void cell_rows(A p, int n, int rows, double* tab, double& zmin, double& r
 }
 
 // core.py:44-79 with unit weights. prof[0..m) is modified (crest). tab is (rows, 2). Returns invert zf and top.
-template <class A>
+template <class A, class T>
 W2D_HD void face_rows(A prof, int m, double crest, double pixel, double n_inv, double length, double e_lo,
-                      double e_hi, int rows, double* tab, double& zf, double& top) {
+                      double e_hi, int rows, T tab, double& zf, double& top) {
     if (!std::isnan(crest))
         for (int q = 0; q < m; ++q) prof[q] = dmax(prof[q], crest);
     zf = prof[0];
@@ -99,10 +100,10 @@ W2D_HD void face_rows(A prof, int m, double crest, double pixel, double n_inv, d
 
 // core.py:82-179: the conveyance column of tab (rows, 2) from the local hydraulic radius over ext[0..n), the face's
 // own pixels being [i0, i1). Scratch: d, per, run_lo, run_hi of n; s0, s1, p0, p1 of n + 1 (run bounds as doubles).
-template <class A>
+template <class A, class T>
 W2D_HD void local_radius_rows(A ext, int n, int i0, int i1, double zf, double top, double pixel, double n_inv,
-                              double length, double beta, int rows, double* tab, A d, A per, A run_lo, A run_hi,
-                              A s0, A s1, A p0, A p1) {
+                              double length, double beta, int rows, T tab, A d, A per, A run_lo, A run_hi, A s0,
+                              A s1, A p0, A p1) {
     const int last = rows - 1;
     const double pix2 = pixel * pixel;
     s0[0] = 0.0, s1[0] = 0.0, p0[0] = 0.0, p1[0] = 0.0;

@@ -95,6 +95,8 @@ _SIGS = {
     "w2d_source_scale": [VP, D],
     "w2d_source_sum": [VP, ctypes.POINTER(D)],
     "w2d_device_count": [ctypes.POINTER(I32)],
+    "w2d_build_tables_device": [VP, I64, I64, I64, I64, D, I32, I32, D, VP, I32, VP, VP, I32,
+                                VP, VP, VP, VP, VP, VP, VP, VP],  # fmt: skip
     "w2d_timing": [VP, I32],
     "w2d_run_setup": [VP, ctypes.POINTER(Member), I32, VP, I64, ctypes.POINTER(RunNumerics), VP],
     "w2d_run_steps": [VP, I32, I32],

@@ -125,8 +125,12 @@ def build_tables(
     manning: float | np.ndarray = 0.035,
     crest: CrestFaces | None = None,
     workers: int | None = None,
+    device: bool = False,
 ) -> GridTables:
-    """All tables of a uniform grid (builder.py:131-258). ``manning`` is one value or a (ny, nx) grid."""
+    """All tables of a uniform grid (builder.py:131-258). ``manning`` is one value or a (ny, nx) grid.
+
+    ``device`` builds them on the GPU (w2d_build_tables_device: the same rows, one thread per cell; a row may
+    differ from the host build by one float32 ulp, the device's cbrt and pow not being the host library's)."""
     cfg = cfg or Tables()
     k = round(grid.dx / bed.pixel)
     if abs(k * bed.pixel - grid.dx) > 1e-6 * grid.dx:
@@ -156,17 +160,23 @@ def build_tables(
     c = _Config(rows, int(cfg.face_conveyance == "local_radius"), cfg.face_depth_min, cfg.face_depth_max,
                 cfg.local_radius_beta, cfg.crest_tolerance)  # fmt: skip
     o = _Out(*[out[n].ctypes.data for n in _OUT_NAMES])
-    lib = _native.host()
-    fn = lib.w2d_build_tables
-    fn.restype = ctypes.c_int
     I64, I32, DBL, VP = ctypes.c_int64, ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
-    fn.argtypes = [VP, I64, I64, I64, I64, DBL, I32, I32, DBL, VP, I32, VP, VP, I32,
-                   VP, VP, VP, VP, VP, VP, ctypes.POINTER(_Config), I32, ctypes.POINTER(_Out)]  # fmt: skip
-    status = fn(
+    argtypes = [VP, I64, I64, I64, I64, DBL, I32, I32, DBL, VP, I32, VP, VP, I32,
+                VP, VP, VP, VP, VP, VP, ctypes.POINTER(_Config)]  # fmt: skip
+    args = [
         _ptr(z), z.shape[0], z.shape[1], bed.row_offset, bed.col_offset, bed.pixel, ny, nx, grid.dx,
         _ptr(tiles), nt, _ptr(mask), _ptr(nman), int(nman_grid), *[_ptr(a) for a in crest_arrays],
-        ctypes.byref(c), table_workers(workers), ctypes.byref(o),
-    )  # fmt: skip
+        ctypes.byref(c),
+    ]  # fmt: skip
+    if device:
+        from wadi2d_amd.solver import hip_backend
+
+        hip_backend.check(hip_backend.api().w2d_build_tables_device(*args, ctypes.byref(o)))
+        return GridTables(grid, **out)
+    fn = _native.host().w2d_build_tables
+    fn.restype = ctypes.c_int
+    fn.argtypes = [*argtypes, I32, ctypes.POINTER(_Out)]
+    status = fn(*args, table_workers(workers), ctypes.byref(o))
     if status != 0:
         raise ValueError("table build refused its arguments")
     return GridTables(grid, **out)
