@@ -117,6 +117,16 @@ int w2d_storage(w2d_handle* h, double* out);
 int w2d_source_weights(w2d_handle* h, const int64_t* index, const double* weight, int64_t n);
 int w2d_source_scale(w2d_handle* h, double scale);
 int w2d_source_sum(w2d_handle* h, double* out);
+/* The source as the rain excess of each cell (forcing.py:644-678 with hydrology/losses.py:49-77): per own cell a
+ * depth factor w (storm.depth_grid's depth, forcing.py:594-601; 1 without one) and a curve number cn (NaN: no losses,
+ * forcing.py:186-203). A window [t0, t1] over which the storm's cumulative depth runs from d0 to d1 mm gives each
+ * cell float((E(d1 w) - E(d0 w)) / denom), denom = 1000 (t1 - t0), where E(P) = P without a curve number, else the
+ * SCS cumulative excess max(P - Ia, 0)^2 / (max(P - Ia, 0) + S), S = (1000 / cn - 10) 25.4 mm, Ia = 0.2 S, NaN
+ * read as 0. Unlisted cells rate 0. The form holds until w2d_source_weights is called. */
+int w2d_source_losses(w2d_handle* h, const int64_t* index, const double* w, const double* cn, int64_t n);
+/* One window's rates into every own cell's src; total = their FP64 sum (in tile order), and tiles[t] (ntiles, or
+ * null) = 1 where a tile has a non-zero rate: the tiles set_source_cells keeps active (model.py:112-122). */
+int w2d_source_excess(w2d_handle* h, double d0, double d1, double denom, double* total, int32_t* tiles);
 /* Kernel timing (no counterpart): with timing on, each step records HIP events on the handle's stream around its
  * face and cell kernels; timing_read waits, and returns and clears the summed milliseconds and the step count. */
 int w2d_timing(w2d_handle* h, int32_t on);
@@ -148,6 +158,11 @@ typedef struct w2d_member_state {
     int64_t nsteps;
     int32_t done, failed, refresh_due;
 } w2d_member_state;
+/* With the excess source (w2d_source_losses), a device-resident run takes (d0, d1, denom) per window, in the order of
+ * the scales (3 nwindows doubles, nwindows = nscales), before w2d_run_setup: its windows' rates are formed on the
+ * device as w2d_source_excess forms them, a window opens when its excess falls or the last one's did, and its kept
+ * tiles are those with a non-zero rate (run.py:1719-1729). */
+int w2d_run_terms(w2d_handle* h, const double* terms, int64_t nwindows);
 /* Set up a run from the handle's current state, active list and the stepper's idle counts (ntiles or null). */
 int w2d_run_setup(w2d_handle* h, const w2d_member* members, int32_t M, const double* scales, int64_t nscales,
                   const w2d_run_numerics* num, const int32_t* idle);

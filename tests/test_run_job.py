@@ -105,7 +105,8 @@ def test_a_reference_shaped_job_with_inert_sections_loads(job_folder):
     (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
     cfg = jobmod.load_job(job_folder / "job.json")
     said = jobmod.ignored(cfg)
-    assert "outputs.live_every_s" in said and "terrain.join_cells" in said and "hydrology" in said
+    assert "outputs.live_every_s" in said and "terrain.join_cells" in said and "hydrology.time_step_min" in said
+    assert not any(x.startswith("hydrology.subbasins") or x.startswith("hydrology.hms") for x in said)
 
 
 @pytest.mark.parametrize(
@@ -296,14 +297,13 @@ def _oracle_run(cfg):
     """The same job through the oracle, in the window loop of run._run (run.py:1712-1739)."""
     from oracle.oracle import Ledger, OracleModel
     from oracle.oracle import Numerics as ONumerics
-    from wadi2d_amd.engine.forcing import Rain, boundary_set
-    from wadi2d_amd.run import boundary_list, build_grid, initial_depth, storm_curve
+    from wadi2d_amd.engine.forcing import boundary_set
+    from wadi2d_amd.run import boundary_list, build_grid, build_rain, initial_depth, storm_curve
     from wadi2d_amd.solver.state import from_state
 
     tables = build_grid(cfg)
     bset = boundary_set(boundary_list(cfg, tables), tables)
-    curve = storm_curve(cfg)
-    rain = Rain(tables, bset, curve if curve is not None else (lambda t: 0.0))
+    rain = build_rain(cfg, tables, bset, storm_curve(cfg))
     m = OracleModel(tables, ONumerics.like(cfg.numerics), prec=4, threads=4)
     m.set_depth(initial_depth(cfg, tables))
     if bset.size:
@@ -344,6 +344,83 @@ def test_a_job_runs_end_to_end_bit_for_bit_the_oracle(job_folder):
     assert (job_folder / "out" / "series" / "lines_gap.csv").exists()
     maps = sorted((job_folder / "out" / "results" / "maps").iterdir())
     assert len(maps) >= 2 and (maps[0] / "depth.npy").exists()
+
+
+def _loss_job(folder: Path, holes: bool = False) -> dict:
+    """The test job with a curve number (80), a CN raster over the west half at cell resolution (a no-data patch
+    falling back to 80, unless ``holes`` drops the default) and a depth-grid storm rising from 20 to 70 mm eastward."""
+    cn = np.full((NY, NX), -9999.0)
+    cn[:, : NX // 2] = 60.0 + 30.0 * np.random.default_rng(4).random((NY, NX // 2))
+    cn[10:20, 5:15] = -9999.0
+    write_asc(folder / "cn.asc", cn, DX, nodata=-9999.0)
+    depth = np.tile(np.linspace(20.0, 70.0, NX), (NY, 1))
+    write_asc(folder / "depth.asc", depth, DX)
+    d = job_dict()
+    d["terrain"]["cn_raster"] = "cn.asc"
+    d["storm"] = {"kind": "depth_grid", "depth_grid": "depth.asc", "duration_min": 20.0}
+    if not holes:
+        d["hydrology"] = {"curve_number": 80.0}
+    (folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    return d
+
+
+def test_curve_numbers_and_a_depth_grid_reach_the_rain(job_folder):
+    from wadi2d_amd.engine.forcing import boundary_set
+    from wadi2d_amd.run import boundary_list, build_grid, build_rain, storm_curve
+
+    _loss_job(job_folder)
+    cfg = jobmod.load_job(job_folder / "job.json")
+    assert Path(cfg.terrain.cn_raster).is_absolute() and Path(cfg.storm.depth_grid).is_absolute()
+    assert not [x for x in jobmod.ignored(cfg) if x.startswith("hydrology")]
+    tables = build_grid(cfg)
+    bset = boundary_set(boundary_list(cfg, tables), tables)
+    rain = build_rain(cfg, tables, bset, storm_curve(cfg))
+    assert rain.excess_form
+    x, _ = tables.grid.xy_of(rain.jj, rain.ii)
+    west = x < X0 + NX * DX / 2
+    assert np.all((rain.cn[west] >= 60.0) & (rain.cn[west] <= 90.0) | (rain.cn[west] == 80.0))
+    assert np.all(rain.cn[~west] == 80.0) and np.any(rain.cn[west] != 80.0)
+    col = np.floor((x - X0) / DX).astype(int)
+    assert np.array_equal(rain.depths, np.linspace(20.0, 70.0, NX)[col].round(4))
+    assert storm_curve(cfg)(600.0) == pytest.approx(0.5)  # the unit storm, half way
+
+
+def test_a_loss_job_with_holes_or_bad_settings_is_refused(job_folder):
+    from wadi2d_amd.engine.forcing import boundary_set
+    from wadi2d_amd.run import boundary_list, build_grid, build_rain, storm_curve
+
+    _loss_job(job_folder, holes=True)
+    cfg = jobmod.load_job(job_folder / "job.json")
+    tables = build_grid(cfg)
+    bset = boundary_set(boundary_list(cfg, tables), tables)
+    with pytest.raises(ValueError, match="cn_raster gives no CN"):
+        build_rain(cfg, tables, bset, storm_curve(cfg))
+    for edit in (lambda d: d.update(hydrology={"curve_number": 0.0}),
+                 lambda d: d["storm"].update(depth_mm=10.0),
+                 lambda d: d.update(storm={"kind": "depth", "depth_mm": 0.0})):  # fmt: skip
+        d = _loss_job(job_folder)
+        edit(d)
+        (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+        with pytest.raises(jobmod.InvalidJob):
+            jobmod.load_job(job_folder / "job.json")
+
+
+@pytest.mark.gpu
+def test_a_job_with_losses_and_a_depth_grid_runs_bit_for_bit_the_oracle(job_folder):
+    from wadi2d_amd.run import run_job
+
+    _loss_job(job_folder)
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "out", backend="hip-exact")
+    print(f"loss run: exit {s.exit_code} steps {s.steps} mass {s.balance.relative:.2e} rain {s.balance.rain:.1f}",
+          flush=True)
+    assert s.exit_code == 0, s.message
+    dmax, bal, steps = _oracle_run(cfg)
+    got = np.load(job_folder / "out" / "results" / "maxima" / "depth.npy")
+    assert s.steps == steps
+    assert np.array_equal(got, dmax.astype(np.float32))
+    assert abs(s.balance.relative) <= 1e-6 and s.balance.rain == pytest.approx(bal.rain, rel=1e-9)
+    assert 0.0 < s.balance.rain
 
 
 @pytest.mark.gpu

@@ -636,6 +636,7 @@ struct RunGlobals {
 struct RunNum {
     double cfl, dt_max, dt_min, wave_floor, dx, area;
     int dt_every, active_every;
+    int excess;  // the source is the rain excess of each cell (excess_kernel), not weight * scale
 };
 
 template <class R>
@@ -655,6 +656,7 @@ struct StepArgs {
     const RunGlobals* glob;
     const double* src_weight;
     double* src_partial;
+    int excess;  // device-resident run with the excess source: excess_kernel has written the window's rates
     // the learned closure's per-face corrections (null when it is off)
     const float2* corr_x;
     const float2* corr_y;
@@ -861,7 +863,7 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
         if (!mb.stepping) return;
         dt_run = (R)mb.dt_step;
         want_run = mb.refresh;
-        new_window = mb.new_window;
+        new_window = mb.new_window && !A.excess;
         scale = mb.scale;
     }
     load_nb3(t, A.st.nbr, nb);
@@ -1200,7 +1202,7 @@ constexpr int PRE_THREADS = 1024;
 // Opens the member's next source window when its clock has reached the last one's end (run.py:1713-1733 with
 // model.py:112-122: new rates, kept tiles, a forced wave refresh), decides this step's refresh and length
 // (stepper.py:105-108), and retires a member that has reached its end or failed.
-__device__ void ctl_begin(Member& b, const double* __restrict__ scales, const RunNum& num, int& any) {
+__device__ void ctl_begin(Member& b, const double* __restrict__ scales, const RunNum& num, double ex_sum, int& any) {
     b.stepping = 0;
     b.new_window = 0;
     b.refresh = 0;
@@ -1220,19 +1222,121 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
         b.window_start = b.window_end;
         b.window_end = fmin(b.window_start + b.window_len, b.t_end);
         b.win = k + 1;
-        if (scale != 0.0 || !b.dry) {
-            b.scale = scale;
-            b.new_window = 1;
-            b.refresh_due = 1;
-            b.rebuild = 1;
-            any = 1;
+        if (num.excess) {
+            // run.py:1719-1729 with forcing.py:667-678: the window's rain is the sum of its cells' excess rates
+            // (excess_kernel), and a window opens when it rains or the last one did; the kept tiles are those with a
+            // non-zero rate (the flags excess_kernel left)
+            const double rain = ex_sum * num.area * (b.window_end - b.window_start);
+            if (rain > 0.0 || !b.dry) {
+                b.scale = 1.0;
+                b.new_window = 1;
+                b.refresh_due = 1;
+                b.rebuild = 1;
+                any = 1;
+            }
+            b.rain_window = rain;
+            b.dry = !(rain > 0.0);
+        } else {
+            if (scale != 0.0 || !b.dry) {
+                b.scale = scale;
+                b.new_window = 1;
+                b.refresh_due = 1;
+                b.rebuild = 1;
+                any = 1;
+            }
+            b.dry = !(scale > 0.0);
         }
-        b.dry = !(scale > 0.0);
     }
     b.refresh = b.refresh_due || (b.nsteps % num.dt_every) == 0;
     b.refresh_due = 0;
     b.dt_step = fmin(fmin(b.dt, b.window_end - b.t), num.dt_max);
     b.stepping = 1;
+}
+
+// Whether ctl_begin opens a window of this member before its next step (the clock has reached the window's end and
+// the member goes on).
+__device__ __forceinline__ bool window_due(const Member& b) {
+    return !b.done && !b.failed && b.t < b.t_end - 1e-9 && b.t >= b.window_end - 1e-9;
+}
+
+// The SCS cumulative excess of depth p, mm (hydrology/losses.py:59-72): S = (1000 / cn - 10) 25.4, Ia = 0.2 S,
+// max(p - Ia, 0)^2 / (max(p - Ia, 0) + S), read through numpy's nan_to_num (forcing.py:659, 664).
+__device__ __forceinline__ double scs_excess(double p, double cn) {
+    const double s = (1000.0 / cn - 10.0) * 25.4;
+    const double ia = 0.2 * s;
+    const double x = p - ia;
+    const double eff = (x >= 0.0 || isnan(x)) ? x : 0.0;  // np.maximum(x, 0.0)
+    const double v = eff * eff / (eff + s);
+    if (isnan(v)) return 0.0;
+    if (isinf(v)) return v > 0.0 ? 1.7976931348623157e308 : -1.7976931348623157e308;
+    return v;
+}
+
+// One cell's rate over a window whose storm depth runs from d0 to d1 mm (forcing.py:644-672): its depth is the
+// storm's times the cell's factor w (a depth grid; 1 without one), its excess that depth where it has no curve
+// number (cn NaN), and the rate float((E1 - E0) / denom), denom = 1000 (t1 - t0).
+template <class R>
+__device__ __forceinline__ R excess_rate(double w, double cn, double d0, double d1, double denom) {
+    double e0 = d0 * w, e1 = d1 * w;
+    if (!isnan(cn)) {
+        e0 = scs_excess(e0, cn);
+        e1 = scs_excess(e1, cn);
+    }
+    return (R)((e1 - e0) / denom);
+}
+
+constexpr int EX_THREADS = 256;
+
+// The excess source of whole tiles: every cell's rate into src, and per tile the FP64 sum of its rates and whether
+// one is non-zero. In a device-resident run (mem set) only the tiles of members that open a window before their
+// next step, with that window's (d0, d1, denom) from terms; in a protocol call (mem null) every tile, with the
+// given window. Each tile is one workgroup's; a tile's cells are all rated, listed or not (forcing.py:667-672 sets
+// every interior cell's rate).
+template <class R>
+__global__ __launch_bounds__(EX_THREADS) void excess_kernel(int nt, const Member* __restrict__ mem,
+                                                          const int* __restrict__ tile_member,
+                                                          const double* __restrict__ terms, double d0a, double d1a,
+                                                          double dna, const double* __restrict__ w,
+                                                          const double* __restrict__ cn, R* __restrict__ src,
+                                                          double* __restrict__ partial, int* __restrict__ flag) {
+    __shared__ double red[EX_THREADS / 64];
+    __shared__ int any_nz;
+    for (int t = 1 + blockIdx.x; t < nt; t += gridDim.x) {
+        double d0 = d0a, d1 = d1a, dn = dna;
+        if (mem) {
+            const Member& b = mem[tile_member[t]];
+            if (t < b.tile0 || t >= b.tile1 || !window_due(b)) continue;
+            if (b.win < b.nwin) {
+                const double* q = terms + 3 * (b.scale_off + b.win);
+                d0 = q[0], d1 = q[1], dn = q[2];
+            } else {
+                d0 = 0.0, d1 = 0.0, dn = 1.0;
+            }
+        }
+        if (threadIdx.x == 0) any_nz = 0;
+        __syncthreads();
+        double sum = 0.0;
+        int nz = 0;
+#pragma unroll
+        for (int r = 0; r < TILE2 / EX_THREADS; ++r) {
+            const size_t c = (size_t)t * TILE2 + r * EX_THREADS + threadIdx.x;
+            const R v = excess_rate<R>(w[c], cn[c], d0, d1, dn);
+            src[c] = v;
+            sum += (double)v;
+            nz |= v != R(0);
+        }
+        for (int off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = sum;
+        if (nz) any_nz = 1;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tot = red[0];
+            for (int q = 1; q < EX_THREADS / 64; ++q) tot += red[q];
+            partial[t] = tot;
+            flag[t] = any_nz;
+        }
+        __syncthreads();
+    }
 }
 
 // Block-wide exclusive prefixes of three flags per thread over a chunk of PRE_THREADS, with one barrier pair:
@@ -1527,7 +1631,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
                                                           const float* __restrict__ hc,
                                                           const float2* __restrict__ ctab, size_t ncell, int last,
                                                           R dx, int fresh, unsigned long long* __restrict__ member_wave,
-                                                          int flush_only) {
+                                                          const double* __restrict__ src_partial, int flush_only) {
     static_assert(PRE_THREADS == TILE2, "transitions give one thread to each cell of a tile");
     __shared__ int wave_cnt[3][PRE_THREADS / 64];
     __shared__ int any;
@@ -1542,9 +1646,23 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         changed = true;
     }
     if (!flush_only) {
+        // the window's rain in the excess form: the sum of the member's tile sums excess_kernel left
+        __shared__ double ex_red[PRE_THREADS / 64];
+        __shared__ int ex_due;
+        if (threadIdx.x == 0) ex_due = num.excess && window_due(b);
+        __syncthreads();
+        double ex_sum = 0.0;
+        if (ex_due) {
+            for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += PRE_THREADS) ex_sum += src_partial[t];
+            for (int off = 32; off > 0; off >>= 1) ex_sum += __shfl_xor(ex_sum, off);
+            if ((threadIdx.x & 63) == 0) ex_red[threadIdx.x >> 6] = ex_sum;
+            __syncthreads();
+            ex_sum = ex_red[0];
+            for (int q = 1; q < PRE_THREADS / 64; ++q) ex_sum += ex_red[q];
+        }
         if (threadIdx.x == 0) {
             int mine = 0;
-            ctl_begin(b, scales, num, mine);
+            ctl_begin(b, scales, num, ex_sum, mine);
             if (b.refresh) member_wave[blockIdx.x] = 0ull;
             any = mine;
         }
@@ -1611,7 +1729,8 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     }
     // the largest wave speed of the member's tiles, gathered by the face kernel's atomic max
     double w = 0.0, rs = 0.0;
-    if (b.new_window) {
+    const bool books_rain = b.new_window && !num.excess;  // the excess form's rain is booked by the pre kernel
+    if (books_rain) {
         for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x)
             if (is_active[t]) rs += src_partial[t];
         for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off);
@@ -1620,11 +1739,11 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     }
     if (threadIdx.x != 0) return;
     if (b.refresh) w = wave_value<R>(member_wave[blockIdx.x]);
-    if (b.new_window) {
+    if (books_rain) {
         rs = red_s[0];
         for (int q = 1; q < POST_THREADS / 64; ++q) rs += red_s[q];
+        b.rain_window = rs * num.area * (b.window_end - b.window_start);
     }
-    if (b.new_window) b.rain_window = rs * num.area * (b.window_end - b.window_start);
     const long long before = b.nsteps;
     b.t += b.dt_step;
     b.nsteps = before + 1;
@@ -1763,6 +1882,13 @@ struct w2d_handle {
     void** run_faces_ptrs = nullptr;  // [parity][cur 4, other 4]
     int run_maxima = 0;
     std::vector<double> weight_host;
+    // the excess form of the source (w2d_source_losses): src_weight holds depth factors, src_cn curve numbers
+    int excess = 0;
+    double* src_cn = nullptr;
+    std::vector<double> run_terms_host;  // (d0, d1, denom) per window of a device-resident run
+    double* run_terms = nullptr;
+    double* ex_partial = nullptr;  // per tile, a protocol call's sums and flags
+    int* ex_flag = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_parity = 0;  // the buffer parity the graph was captured at; it replays only at that parity
@@ -2529,6 +2655,7 @@ int w2d_source_weights(w2d_handle* H, const int64_t* index, const double* w, int
         own[c] = w[m];
     }
     H->weight_host = own;
+    H->excess = 0;
     if (!H->src_weight) {
         int s = dalloc(H, (void**)&H->src_weight, H->ncell * sizeof(double));
         if (s) return s;
@@ -2540,6 +2667,7 @@ int w2d_source_weights(w2d_handle* H, const int64_t* index, const double* w, int
 
 int w2d_source_scale(w2d_handle* H, double scale) {
     if (!H || !H->src_weight) return fail(1, "source_scale: call source_weights first");
+    if (H->excess) return fail(1, "source_scale: the source is in the excess form (w2d_source_losses)");
     if (H->prec == 8)
         scale_source_kernel<double><<<1024, 256, 0, H->stream>>>(H->ncell, H->src_weight, scale, (double*)H->src);
     else
@@ -2552,6 +2680,68 @@ int w2d_source_sum(w2d_handle* H, double* out) {
     if (!H || !out) return fail(1, "source_sum: bad arguments");
     if (H->prec == 8) return big_sum<double>(H, (int64_t)H->ncell, nullptr, (double*)H->src, nullptr, out);
     return big_sum<float>(H, (int64_t)H->ncell, nullptr, (float*)H->src, nullptr, out);
+}
+
+int w2d_source_losses(w2d_handle* H, const int64_t* index, const double* w, const double* cn, int64_t n) {
+    if (!H || n < 0 || (n && (!index || !w || !cn))) return fail(1, "source_losses: bad arguments");
+    std::vector<double> own_w(H->ncell, 0.0), own_cn(H->ncell, std::nan(""));
+    for (int64_t m = 0; m < n; ++m) {
+        const int64_t c = own_of(H, index[m]);
+        if (c < 0) return fail(1, "source_losses: an index is not an own cell");
+        if (!std::isfinite(w[m])) return fail(1, "source_losses: a depth factor is not finite");
+        if (!std::isnan(cn[m]) && !(cn[m] > 0.0 && cn[m] <= 100.0))
+            return fail(1, "source_losses: a curve number is outside (0, 100]");
+        own_w[c] = w[m];
+        own_cn[c] = cn[m];
+    }
+    int s;
+    if (!H->src_weight && (s = dalloc(H, (void**)&H->src_weight, H->ncell * sizeof(double)))) return s;
+    if (!H->src_cn && (s = dalloc(H, (void**)&H->src_cn, H->ncell * sizeof(double)))) return s;
+    const size_t nt = (size_t)H->ntiles;
+    if (!H->ex_partial && (s = dalloc(H, (void**)&H->ex_partial, nt * sizeof(double)))) return s;
+    if (!H->ex_flag && (s = dalloc(H, (void**)&H->ex_flag, nt * sizeof(int)))) return s;
+    HIP_TRY(hipMemcpyAsync(H->src_weight, own_w.data(), H->ncell * sizeof(double), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipMemcpyAsync(H->src_cn, own_cn.data(), H->ncell * sizeof(double), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    H->weight_host = own_w;
+    H->excess = 1;
+    return 0;
+}
+
+int w2d_source_excess(w2d_handle* H, double d0, double d1, double denom, double* total, int32_t* tiles) {
+    if (!H || !total || !(denom > 0.0)) return fail(1, "source_excess: bad arguments");
+    if (!H->excess) return fail(1, "source_excess: call source_losses first");
+    const int nt = H->ntiles;
+    if (H->prec == 8)
+        excess_kernel<double><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
+            nt, nullptr, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (double*)H->src, H->ex_partial,
+            H->ex_flag);
+    else
+        excess_kernel<float><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
+            nt, nullptr, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (float*)H->src, H->ex_partial,
+            H->ex_flag);
+    HIP_TRY(hipGetLastError());
+    std::vector<double> part(nt, 0.0);
+    std::vector<int32_t> flag(nt, 0);
+    HIP_TRY(hipMemcpyAsync(part.data(), H->ex_partial, nt * sizeof(double), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipMemcpyAsync(flag.data(), H->ex_flag, nt * sizeof(int), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    double sum = 0.0;
+    for (int t = 1; t < nt; ++t) sum += part[t];
+    *total = sum;
+    if (tiles) {
+        tiles[0] = 0;
+        for (int t = 1; t < nt; ++t) tiles[t] = flag[t];
+    }
+    return 0;
+}
+
+int w2d_run_terms(w2d_handle* H, const double* terms, int64_t nwindows) {
+    if (!H || nwindows < 0 || (nwindows && !terms)) return fail(1, "run_terms: bad arguments");
+    for (int64_t k = 0; k < nwindows; ++k)
+        if (!(terms[3 * k + 2] > 0.0)) return fail(1, "run_terms: a window's 1000 (t1 - t0) is not positive");
+    H->run_terms_host.assign(terms, terms + 3 * nwindows);
+    return 0;
 }
 
 int w2d_timing(w2d_handle* H, int32_t on) {
@@ -2711,7 +2901,8 @@ int enqueue_pre(w2d_handle* H, int flush_only) {
     R* const* fc = (R* const*)(H->run_faces_ptrs + 8 * H->ucur);  // current u, v, mx, my, then the others
     pre_kernel<R, FAST><<<H->run_m, PRE_THREADS, 0, H->stream>>>(
         H->run_mem, H->run_glob, H->run_scales, H->run_num, run_lists(H), (R*)H->h[H->hcur], (R*)H->h[1 - H->hcur],
-        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, (R)H->P.dx, H->m_fresh, H->run_wave, flush_only);
+        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, (R)H->P.dx, H->m_fresh, H->run_wave,
+        H->run_src_partial, flush_only);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2725,9 +2916,16 @@ int enqueue_run_step(w2d_handle* H) {
     A.src_weight = H->src_weight;
     A.src_partial = H->run_src_partial;
     A.member_wave = H->run_wave;
+    A.excess = H->excess;
     const int nt = H->ntiles;
     R* h_new = (R*)H->h[1 - H->hcur];
     int s;
+    if (H->excess) {
+        excess_kernel<R><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
+            nt, H->run_mem, H->run_tile_member, H->run_terms, 0.0, 0.0, 1.0, H->src_weight, H->src_cn, (R*)H->src,
+            H->run_src_partial, H->run_has_source);
+        HIP_TRY(hipGetLastError());
+    }
     if ((s = enqueue_pre<R, FAST>(H, 0))) return s;
     if (H->timing && (s = mark(H))) return s;
     if ((s = launch_faces<R, FAST>(H, A, nt))) return s;
@@ -2799,7 +2997,9 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     // tiles that keep themselves active: a source, a boundary cell (model.py:117-122, 145-150); each member's
     // boundary cells and tally faces must be one run of the lists, in member order
     std::vector<int32_t> has_source(nt, 0), bnd_tile(nt, 0), is_active(nt, 0);
-    if (!H->weight_host.empty())
+    if (H->excess && H->run_terms_host.size() != 3 * (size_t)nscales)
+        return fail(1, "run_setup: the excess source needs w2d_run_terms for every window first");
+    if (!H->excess && !H->weight_host.empty())
         for (size_t c = 0; c < H->ncell; ++c)
             if (H->weight_host[c] != 0.0) has_source[c / TILE2] = 1;
     auto ranges = [&](const int* dev, int n, bool cells, const char* what) -> int {
@@ -2896,7 +3096,17 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         if ((s = upload(H, (void***)&H->run_faces_ptrs, (void**)ptrs, 16))) return s;
     }
     H->run_num = RunNum{num->cfl, num->dt_max, num->dt_min, num->wave_floor, H->P.dx, H->P.dx * H->P.dx,
-                        num->dt_every, num->active_every};
+                        num->dt_every, num->active_every, H->excess};
+    if (H->excess) {
+        if (H->run_terms) {
+            H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_terms), H->owned.end());
+            (void)hipFree(H->run_terms);
+            H->run_terms = nullptr;
+        }
+        std::vector<double> tv = H->run_terms_host;
+        if (tv.empty()) tv.assign(3, 0.0);
+        if ((s = upload(H, &H->run_terms, tv.data(), tv.size()))) return s;
+    }
     if (num->maxima < 0) return fail(1, "run_setup: maxima is an interval in steps (0: none)");
     H->run_maxima = num->maxima;
     if (H->run_maxima && !H->maxima[0]) return fail(1, "run_setup: maxima asked for but maxima_set not called");

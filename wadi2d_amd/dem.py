@@ -81,7 +81,7 @@ def read_raster(path: str | Path, stride: int = 1) -> Raster:
     """The ground of ``path``; ``stride`` reads every n-th pixel (terrain.pixel_stride)."""
     p = Path(path)
     if not p.is_file():
-        raise FileNotFoundError(f"DEM not found: {p}")
+        raise FileNotFoundError(f"raster not found: {p}")
     suffix = p.suffix.lower()
     if suffix == ".asc":
         r = _read_asc(p)
@@ -128,3 +128,20 @@ def grid_over(r: Raster, dx: float, extent: tuple[float, float, float, float] | 
     z = np.where(np.isfinite(z), z, fill)
     bed = ArrayBed(z, r.pixel, row_offset=r0 - k, col_offset=c0 - k)
     return grid, bed
+
+
+def raster_at_cells(path: str | Path, grid) -> np.ndarray:
+    """A raster read at the centres of the grid's cells (terrain/sampling.py:28-70): the value of the pixel holding
+    each mask cell's centre, NaN off the raster, on no-data and off the mask. The raster must share the grid's
+    coordinate system (this path reads no CRS)."""
+    r = read_raster(path)
+    out = np.full(grid.shape, np.nan)
+    jj, ii = np.nonzero(grid.mask)
+    if jj.size == 0:
+        return out
+    x, y = grid.xy_of(jj, ii)
+    cols = np.floor((x - r.x0) / r.pixel).astype(np.int64)
+    rows = np.floor((r.y0 - y) / r.pixel).astype(np.int64)
+    on = (cols >= 0) & (cols < r.z.shape[1]) & (rows >= 0) & (rows < r.z.shape[0])
+    out[jj[on], ii[on]] = np.asarray(r.z, np.float64)[rows[on], cols[on]]
+    return out

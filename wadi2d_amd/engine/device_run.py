@@ -5,7 +5,9 @@ captured graphs of steps with no host in the loop.
 Each member keeps the reference's sequence: a window opens every ``source_every_s`` (new rates, kept tiles, a
 forced wave refresh), a step is ``min(dt, window end - t, dt_max)``, the wave speed is reduced every ``dt_every``
 steps into the next ``dt``, the active list is checked every ``active_every`` steps. The window's source rates are
-its precomputed scale times each cell's weight, float(weight * scale), as ``Rain.rate`` forms them on the host.
+its precomputed scale times each cell's weight, float(weight * scale), as ``Rain.rate`` forms them on the host; in
+the excess form (curve numbers, a depth grid) the device forms every cell's rate from the window's storm depths
+when the window is due, and the window's rain decides whether it opens.
 """
 
 from __future__ import annotations
@@ -46,6 +48,17 @@ def window_scales(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarr
     return np.asarray(out, np.float64)
 
 
+def window_terms(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarray:
+    """(d0, d1, 1000 (t1 - t0)) of every window of [t0, t_end]: the excess form's windows as the device takes them."""
+    out = []
+    t = t0
+    while t < t_end - 1e-9:
+        t_next = min(t + every, t_end)
+        out.append(rain.terms(t, t_next))
+        t = t_next
+    return np.asarray(out, np.float64).reshape(-1, 3)
+
+
 class DeviceRun:
     def __init__(self, backend, plans: list[MemberPlan], num, source_every_s: float = 30.0, maxima: int = 0,
                  idle: np.ndarray | None = None) -> None:  # fmt: skip
@@ -62,6 +75,11 @@ class DeviceRun:
             scales.append(sc)
             off += sc.size
         self.scales = np.concatenate(scales) if scales else np.zeros(0)
+        excess = {p.rain.excess_form for p in plans}
+        if len(excess) > 1:
+            raise ValueError("a device run's members take one source form: all weighted, or all in the excess form")
+        if excess == {True}:
+            backend.run_terms(np.concatenate([window_terms(p.rain, p.t0, p.t_end, source_every_s) for p in plans]))
         rn = RunNumerics(num.cfl, num.dt_max, num.dt_min, num.wave_floor, num.dt_every, num.active_every, int(maxima))
         backend.run_setup(members, self.scales, rn, idle)
         self.state = backend.run_read()

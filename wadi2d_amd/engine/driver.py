@@ -54,9 +54,15 @@ class Driver:
         self.area = model.grid.dx**2
         if boundary.size:
             model.set_boundary(boundary.cells)
-        weight = rain.weight if rain.weight is not None else np.ones(rain.state.size)
-        self.b.source_weights(rain.state, weight)
-        self.source_tiles = np.unique(rain.state[weight != 0.0] // STRIDE2)
+        if rain.excess_form:
+            n = rain.state.size
+            self.b.source_losses(rain.state, rain.depths if rain.depths is not None else np.ones(n),
+                                 rain.cn if rain.cn is not None else np.full(n, np.nan))  # fmt: skip
+            self.source_tiles = None
+        else:
+            weight = rain.weight if rain.weight is not None else np.ones(rain.state.size)
+            self.b.source_weights(rain.state, weight)
+            self.source_tiles = np.unique(rain.state[weight != 0.0] // STRIDE2)
         faces = np.concatenate((boundary.xfaces, boundary.yfaces)).astype(np.int64)
         axis = np.concatenate((np.zeros(boundary.xfaces.size), np.ones(boundary.yfaces.size))).astype(np.int32)
         sign = np.concatenate((boundary.xsign, boundary.ysign)).astype(np.float32)
@@ -96,6 +102,9 @@ class Driver:
 
     def open_window(self, t: float, t_next: float) -> None:
         """The source of window [t, t_next]: its rate on the device, its tiles kept, a forced refresh, its rain."""
+        if self.rain.excess_form:
+            self._open_excess(t, t_next)
+            return
         s = self.rain.scale(t, t_next)
         if s != 0.0 or not self.dry:
             self.b.source_scale(s)
@@ -108,6 +117,22 @@ class Driver:
             st.rebuild_active(retain=True)
             st.refresh_due = True
         rain = self.b.source_sum() * self.area * (t_next - t) if s != 0.0 else 0.0
+        self.dry = rain <= 0.0
+        self.window = (t, t_next, rain)
+        self.settled = t
+
+    def _open_excess(self, t: float, t_next: float) -> None:
+        """The excess form (run.py:1719-1729): every cell's rate formed on the device; the window opens (kept tiles:
+        those with a non-zero rate, model.py:112-122) when its rain is positive or the last window's was."""
+        total, tiles = self.b.source_excess(*self.rain.terms(t, t_next))
+        rain = total * self.area * (t_next - t)
+        if rain > 0.0 or not self.dry:
+            m, st = self.model, self.model.stepper
+            keep = st.keep_active
+            keep[:] = tiles
+            keep[m.boundary_tiles] = True
+            st.rebuild_active(retain=True)
+            st.refresh_due = True
         self.dry = rain <= 0.0
         self.window = (t, t_next, rain)
         self.settled = t

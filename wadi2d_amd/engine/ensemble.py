@@ -109,7 +109,9 @@ def stack_tables(members: list[GridTables]) -> tuple[GridTables, list[Slot]]:
 
 class MemberRain:
     """A member's storm (cumulative depth in mm against time in s) over its interior cells (state indices of the
-    mosaic): the window scale the device forms rates from, as ``Rain.scale``."""
+    mosaic): the window scale the device forms rates from, as ``Rain.scale`` (the weight form)."""
+
+    excess_form = False
 
     def __init__(self, storm, state: np.ndarray) -> None:
         self.storm = storm

@@ -12,6 +12,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from wadi2d_amd.config import MM_PER_M
+from wadi2d_amd.engine.losses import cell_excess
 from wadi2d_amd.solver.backend import BoundaryCells
 from wadi2d_amd.solver.state import state_index
 from wadi2d_amd.tables import hbar_at_level
@@ -123,32 +124,67 @@ def boundary_set(boundaries: list[Boundary], tables) -> BoundarySet:
 
 
 class Rain:
-    """A storm as a cumulative depth curve times a per-cell weight (1 for a uniform storm), over the interior cells.
+    """A storm as a cumulative depth curve over the interior cells, in one of two forms.
 
-    ``rate(t0, t1)`` is the reference's window mean (forcing.py:667-678): the difference of cumulative depth over
-    the window, in m/s, taken to float32 per cell, and the FP64 volume of the window from those float32 rates.
+    * weight form: the curve times a per-cell weight (1 for a uniform storm). ``rate(t0, t1)`` is the window's mean
+      rate per unit weight, ``scale``, times the weight, taken to float32 per cell.
+    * excess form (``curve_number`` or ``depth_grid`` given): the reference's own (forcing.py:644-678). A cell's
+      cumulative depth is the curve's times its depth factor (storm.depth_grid; the curve alone without one), its
+      cumulative excess that depth through the SCS loss of its curve number (NaN: none), and its rate the window's
+      difference of excess over 1000 (t1 - t0), taken to float32.
+
+    Either way the window's rain volume is the FP64 sum of the float32 rates times the cell area and the window.
     """
 
-    def __init__(self, tables, boundary: BoundarySet, cumulative_mm, weight: np.ndarray | None = None) -> None:
+    def __init__(self, tables, boundary: BoundarySet, cumulative_mm, weight: np.ndarray | None = None,
+                 curve_number: np.ndarray | None = None, depth_grid: np.ndarray | None = None) -> None:  # fmt: skip
         grid = tables.grid
         interior = grid.mask & ~boundary.mask
         self.jj, self.ii = np.nonzero(interior)
         self.state = np.asarray(state_index(grid, self.jj, self.ii), np.int64)
         self.area = grid.dx**2
         self.cumulative_mm = cumulative_mm
+        if weight is not None and (curve_number is not None or depth_grid is not None):
+            raise ValueError("a weighted storm takes no curve numbers or depth grid: give those as the excess form")
         self.weight = None if weight is None else np.asarray(weight, np.float64)[self.jj, self.ii]
+        self.cn = None if curve_number is None else np.asarray(curve_number, np.float64)[self.jj, self.ii]
+        self.depths = None if depth_grid is None else np.asarray(depth_grid, np.float64)[self.jj, self.ii]
+        if self.cn is not None and np.any(np.isfinite(self.cn) & ~((self.cn > 0.0) & (self.cn <= 100.0))):
+            raise ValueError("a curve number lies outside (0, 100]")
+        if self.depths is not None and not np.all(np.isfinite(self.depths)):
+            raise ValueError("a depth-grid value is missing on a rained cell")
+
+    @property
+    def excess_form(self) -> bool:
+        return self.cn is not None or self.depths is not None
 
     def scale(self, t0: float, t1: float) -> float:
         """The window's mean rate per unit weight, m/s (float64)."""
         return (float(self.cumulative_mm(t1)) - float(self.cumulative_mm(t0))) / (MM_PER_M * (t1 - t0))
 
+    def terms(self, t0: float, t1: float) -> tuple[float, float, float]:
+        """The excess form's window as the device takes it: the storm depth at both ends, and 1000 (t1 - t0)."""
+        return float(self.cumulative_mm(t0)), float(self.cumulative_mm(t1)), MM_PER_M * (t1 - t0)
+
+    def excess_mm(self, t: float) -> np.ndarray:
+        """Cumulative excess per interior cell at time t (forcing.py:644-665)."""
+        depth = float(self.cumulative_mm(t))
+        rain = depth * self.depths if self.depths is not None else np.full(self.state.size, depth)
+        if self.cn is None:
+            return rain
+        return cell_excess(rain, self.cn)
+
     def rate(self, t0: float, t1: float) -> tuple[np.ndarray, float]:
-        s = self.scale(t0, t1)
-        if self.weight is None:
-            out = np.full(self.state.size, np.float32(s), np.float32)
+        step = t1 - t0
+        if self.excess_form:  # forcing.py:667-672
+            out = ((self.excess_mm(t1) - self.excess_mm(t0)) / (MM_PER_M * step)).astype(np.float32)
         else:
-            out = (self.weight * s).astype(np.float32)
-        rain = float(out.sum(dtype=np.float64)) * self.area * (t1 - t0)
+            s = self.scale(t0, t1)
+            if self.weight is None:
+                out = np.full(self.state.size, np.float32(s), np.float32)
+            else:
+                out = (self.weight * s).astype(np.float32)
+        rain = float(out.sum(dtype=np.float64)) * self.area * step
         return out, rain
 
 

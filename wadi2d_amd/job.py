@@ -63,6 +63,8 @@ class Terrain(_Part):
     pixel_stride: int = Field(1, ge=1, le=64, description="Read every n-th DEM pixel")
     manning: float = Field(0.035, ge=0.008, le=0.5, description="Default Manning n, s/m^(1/3)")
     manning_raster: str | None = None
+    cn_raster: str | None = Field(None, description="Curve numbers read at the cell centres (.asc or .npz); they "
+                                  "beat hydrology.curve_number where they have a value (forcing.py:186-203)")
 
 
 class Storm(_Part):
@@ -78,9 +80,16 @@ class Storm(_Part):
     arf_factor: float | None = Field(None, gt=0.0, le=1.0)
     hyetograph_min: list[float] | None = None
     hyetograph_mm: list[float] | None = None
+    depth_grid: str | None = Field(None, description="Storm depths per cell, mm (.asc or .npz), for kind "
+                                   "'depth_grid': each cell takes its depth over duration_min (forcing.py:594-601)")
 
     @model_validator(mode="after")
     def _complete(self) -> Storm:
+        # schema.py:1158-1161
+        if self.kind == "depth" and (self.depth_grid is not None or self.depth_mm <= 0.0):
+            raise ValueError("a depth storm needs storm.depth_mm above zero; a raster of depths is kind 'depth_grid'")
+        if self.kind == "depth_grid" and (self.depth_grid is None or self.depth_mm != 0.0):
+            raise ValueError("a depth_grid storm needs storm.depth_grid, and takes its depths from it, not depth_mm")
         if self.kind == "hypothetical":
             t, d = self.time_percent, self.depth_percent
             if not t or not d or len(t) != len(d):
@@ -141,9 +150,9 @@ class Run(_Part):
 # (section, key): settings that change the water and that this path does not run; a job must leave them off
 SWITCHED_OFF = (
     ("grid", "domain"), ("grid", "exclude_lumped"), ("grid", "exclude_subbasins"),
-    ("terrain", "manning_raster"), ("terrain", "cn_raster"), ("terrain", "objects"), ("terrain", "roughness"),
+    ("terrain", "manning_raster"), ("terrain", "objects"), ("terrain", "roughness"),
     ("terrain", "fill_pits"),
-    ("storm", "depth_grid"), ("storm", "rain_file"), ("storm", "gauges"), ("storm", "area_reduction"),
+    ("storm", "rain_file"), ("storm", "gauges"), ("storm", "area_reduction"),
     ("storm", "arf"), ("storm", "pattern"),
     ("numerics", "local_time_stepping"),
     ("mesh", "enabled"), ("strips", "enabled"), ("transmission", "enabled"),
@@ -201,6 +210,14 @@ class RunConfig(BaseModel):
                 data[name] = known
         return data
 
+    @model_validator(mode="after")
+    def _curve_number(self) -> RunConfig:
+        """hydrology.curve_number: CN of 2D cells without a better source, in (0, 100] (schema.py:1435-1437)."""
+        cn = self.hydrology.get("curve_number")
+        if cn is not None and (isinstance(cn, bool) or not isinstance(cn, (int, float)) or not 0.0 < cn <= 100.0):
+            raise ValueError("hydrology.curve_number must lie in (0, 100]")
+        return self
+
 
 def unsupported(cfg: RunConfig) -> list[str]:
     """What the job switches on that this path does not run, in words; empty for a job it runs as written."""
@@ -215,7 +232,7 @@ def unsupported(cfg: RunConfig) -> list[str]:
     for name in TOP_LISTS_OFF:
         if getattr(cfg, name):
             out.append(name)
-    if cfg.storm.kind in ("gridded", "gauges", "depth_grid"):
+    if cfg.storm.kind in ("gridded", "gauges"):
         out.append(f"storm.kind '{cfg.storm.kind}'")
     out += [f"boundaries: kind 'flow' ({b.name or 'unnamed'})" for b in cfg.boundaries if b.kind == "flow"]
     return out
@@ -228,11 +245,15 @@ def ignored(cfg: RunConfig) -> list[str]:
     for name in ("grid", "terrain", "storm", "outputs", "run"):
         extra = getattr(cfg, name).model_extra or {}
         out += [f"{name}.{k}" for k, v in extra.items() if not _falsy(v)]
-    out += [name for name in READ_ONLY_SECTIONS if getattr(cfg, name)]
+    out += [name for name in READ_ONLY_SECTIONS if name != "hydrology" and getattr(cfg, name)]
+    out += [f"hydrology.{k}" for k, v in cfg.hydrology.items() if k not in RUN_HYDROLOGY and not _falsy(v)]
     return out
 
 
-PATH_FIELDS = (("terrain", "dem"),)
+RUN_HYDROLOGY = ("curve_number",)
+"""Keys of the hydrology section this path runs: the 2D cells' default curve number."""
+
+PATH_FIELDS = (("terrain", "dem"), ("terrain", "cn_raster"), ("storm", "depth_grid"))
 """Paths of a job this path reads; relative ones are relative to the job file (run.py:147-163, 209-220)."""
 
 

@@ -23,7 +23,7 @@ from pathlib import Path
 
 import numpy as np
 
-from wadi2d_amd.dem import grid_over, read_raster
+from wadi2d_amd.dem import grid_over, raster_at_cells, read_raster
 from wadi2d_amd.engine import forcing
 from wadi2d_amd.engine.driver import Balance, Driver
 from wadi2d_amd.engine.lines import cells_along, inside, line_faces
@@ -40,6 +40,8 @@ from wadi2d_amd.job import BackendUnavailable, InvalidJob, JobError, RunConfig, 
 from wadi2d_amd.solver.state import from_state, state_index
 from wadi2d_amd.solver.stepper import NumericalFailure
 from wadi2d_amd.tables import build_tables, hbar_at_level
+
+UNIT_DEPTH = 1.0  # forcing.py:54: a depth grid's storm falls one millimetre, each cell scaling it by its own depth
 
 EXIT_DONE, EXIT_UNEXPECTED, EXIT_INVALID, EXIT_INPUT, EXIT_BACKEND = 0, 1, 2, 3, 4
 EXIT_NUMERICAL, EXIT_STOPPED, EXIT_MEMORY = 5, 6, 7
@@ -85,6 +87,8 @@ def storm_curve(cfg: RunConfig):
         return None
     if s.kind in ("uniform", "depth"):
         return forcing.uniform_storm(s.depth_mm * factor, s.duration_min * 60.0)
+    if s.kind == "depth_grid":  # a unit depth, each cell's own from the grid (forcing.py:109)
+        return forcing.uniform_storm(UNIT_DEPTH * factor, s.duration_min * 60.0)
     if s.kind == "hypothetical":
         tp = np.concatenate(([0.0], np.asarray(s.time_percent, np.float64)))
         dp = np.concatenate(([0.0], np.asarray(s.depth_percent, np.float64)))
@@ -108,6 +112,36 @@ def storm_curve(cfg: RunConfig):
 
         return hyetograph
     raise InvalidJob(f"storm.kind {s.kind!r} is not run by this path")
+
+
+def build_rain(cfg: RunConfig, tables, bset, curve) -> forcing.Rain:
+    """The job's rain over the interior cells (forcing.py:527-550): with a curve number (hydrology.curve_number,
+    beaten by terrain.cn_raster where it has a value, forcing.py:186-203) or a depth grid, the excess form, whose
+    rates the device forms; else the storm curve alone."""
+    grid = tables.grid
+    interior = grid.mask & ~bset.mask
+    default = cfg.hydrology.get("curve_number")
+    cn = None
+    if default is not None or cfg.terrain.cn_raster is not None:
+        cn = np.full(grid.shape, np.nan if default is None else float(default))
+        if cfg.terrain.cn_raster is not None:
+            raster = raster_at_cells(cfg.terrain.cn_raster, grid)
+            cn = np.where(np.isfinite(raster), raster, cn)
+            holes = int(np.count_nonzero(np.isnan(cn) & interior))
+            if holes:  # forcing.py:582-592
+                raise ValueError(f"terrain.cn_raster gives no CN on {holes} rained cells of {int(interior.sum())}, "
+                                 "and nothing else does: extend the raster or set hydrology.curve_number")
+        if np.any(np.isfinite(cn) & interior & ~((cn > 0.0) & (cn <= 100.0))):
+            raise ValueError("terrain.cn_raster holds curve numbers outside (0, 100]")
+    depths = None
+    if cfg.storm.kind == "depth_grid":  # forcing.py:594-601
+        values = raster_at_cells(cfg.storm.depth_grid, grid)
+        holes = int(np.count_nonzero(~np.isfinite(values) & interior))
+        if holes:
+            raise ValueError(f"storm.depth_grid has no depth on {holes} rained cells of {int(interior.sum())}")
+        depths = np.nan_to_num(values)
+    zero = lambda t: 0.0  # noqa: E731
+    return forcing.Rain(tables, bset, curve if curve is not None else zero, curve_number=cn, depth_grid=depths)
 
 
 def build_grid(cfg: RunConfig):
@@ -335,7 +369,7 @@ def _run(cfg: RunConfig, out: Path, backend: str | None, resume: bool, progress:
     summary.backend = model.backend.name if hasattr(model.backend, "name") else name
     bset = forcing.boundary_set(boundary_list(cfg, tables), tables)
     model.set_depth(initial_depth(cfg, tables))
-    rain = forcing.Rain(tables, bset, curve if curve is not None else (lambda t: 0.0))
+    rain = build_rain(cfg, tables, bset, curve)
     drv = Driver(model, bset, rain, cfg.run.source_every_s, maxima_every=cfg.outputs.maxima_every)
     model.backend.maxima_set(cfg.outputs.arrival_depth, cfg.outputs.speed_depth)
     results = Results(out / "results", grid, cfg.model_dump_json(), append=saved is not None)

@@ -94,6 +94,9 @@ _SIGS = {
     "w2d_source_weights": [VP, VP, VP, I64],
     "w2d_source_scale": [VP, D],
     "w2d_source_sum": [VP, ctypes.POINTER(D)],
+    "w2d_source_losses": [VP, VP, VP, VP, I64],
+    "w2d_source_excess": [VP, D, D, D, ctypes.POINTER(D), VP],
+    "w2d_run_terms": [VP, VP, I64],
     "w2d_device_count": [ctypes.POINTER(I32)],
     "w2d_build_tables_device": [VP, I64, I64, I64, I64, D, I32, I32, D, VP, I32, VP, VP, I32,
                                 VP, VP, VP, VP, VP, VP, VP, VP],  # fmt: skip
@@ -328,6 +331,26 @@ class HipBackend:
         out = D(0.0)
         check(api().w2d_source_sum(self.h, ctypes.byref(out)))
         return float(out.value)
+
+    def source_losses(self, index: np.ndarray, depth_factor: np.ndarray, curve_number: np.ndarray) -> None:
+        """The excess form of the source: a depth factor and a curve number (NaN: none) per own cell."""
+        idx, w, cn = _c(index, np.int64), _c(depth_factor, np.float64), _c(curve_number, np.float64)
+        if not (idx.size == w.size == cn.size):
+            raise ValueError("source_losses: index, depth factor and curve number differ in length")
+        check(api().w2d_source_losses(self.h, idx.ctypes.data, w.ctypes.data, cn.ctypes.data, int(idx.size)))
+
+    def source_excess(self, d0: float, d1: float, denom: float) -> tuple[float, np.ndarray]:
+        """One window's excess rates on the device: their FP64 sum and the per-tile non-zero flags."""
+        total = D(0.0)
+        tiles = np.zeros(self.grid.ntiles, np.int32)
+        check(api().w2d_source_excess(self.h, float(d0), float(d1), float(denom), ctypes.byref(total),
+                                      tiles.ctypes.data))  # fmt: skip
+        return float(total.value), tiles.astype(bool)
+
+    def run_terms(self, terms: np.ndarray) -> None:
+        """(d0, d1, 1000 (t1 - t0)) of every window of a device run, as (nwindows, 3)."""
+        t = _c(np.asarray(terms).reshape(-1, 3), np.float64)
+        check(api().w2d_run_terms(self.h, t.ctypes.data if t.size else None, int(t.shape[0])))
 
     def timing(self, on: bool) -> None:
         check(api().w2d_timing(self.h, int(on)))
