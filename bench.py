@@ -68,6 +68,11 @@ def main() -> int:
     ap.add_argument("--workload", choices=("step", "ensemble", "precision", "learned"), default="step",
                     help="step: the 4096x4096 plain step (default); ensemble: the 64-member storm/ARF ensemble; "
                          "precision: FP32 against FP64 on the steep 2048x2048 reservoir break")  # fmt: skip
+    ap.add_argument("--tables", choices=("host", "device"), default="host",
+                    help="step: build the sub-grid tables on the host (OpenMP) or on the GPU")
+    ap.add_argument("--losses", choices=("off", "on"), default="off",
+                    help="step: the storm as a depth grid with SCS curve-number losses (CN 60-95), rates formed on "
+                         "the device each window")  # fmt: skip
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
     ap.add_argument("--serial-members", type=int, default=-1, help="members the serial baseline runs (-1: all)")
     ap.add_argument("--reference", choices=("on", "off"), default="on", help="the FP64 batched run (ensemble)")
@@ -102,10 +107,18 @@ def main() -> int:
     scen = synth.full_extent_case(args.size)
     log(f"[bench] ground {scen.name}: {time.time() - t0:.1f} s")
     t0 = time.time()
+    if args.tables == "device":
+        from wadi2d_amd.tables import build_tables
+
+        scen.tables = build_tables(scen.grid, scen.bed, scen.tables_cfg, scen.manning, device=True)
     tables = scen.build()
-    log(f"[bench] tables: {time.time() - t0:.1f} s, {tables.grid.ntiles - 1} tiles")
+    log(f"[bench] tables ({args.tables}): {time.time() - t0:.1f} s, {tables.grid.ntiles - 1} tiles")
     bset = boundary_set(synth.edge_outlets(scen), tables)
-    rain = Rain(tables, bset, scen.cumulative_mm, scen.weight)
+    if args.losses == "on":
+        cn = 60.0 + 35.0 * np.random.default_rng(17).random(tables.grid.shape)
+        rain = Rain(tables, bset, scen.cumulative_mm, curve_number=cn, depth_grid=scen.weight)
+    else:
+        rain = Rain(tables, bset, scen.cumulative_mm, scen.weight)
     del scen.bed
     backend = "hip" if args.variant == "fast" else "hip-exact"
     t0 = time.time()
@@ -191,7 +204,9 @@ def main() -> int:
         "data": "synthetic (seeded ground, tables and storm built in-process)",
         "config": {
             "workload": f"plain FP32 step, {args.size}x{args.size} cells at 30 m, 12-row sub-grid tables, hot start "
-                        f"at t={hot:.0f} s of a Gaussian 80 mm/h storm, four free outflows",
+                        f"at t={hot:.0f} s of a Gaussian 80 mm/h storm, four free outflows"
+                        + (", the storm as a depth grid with SCS losses (CN 60-95) formed on the device"
+                           if args.losses == "on" else ""),
             "cells": domain_cells,
             "active_tiles": tiles,
             "variant": args.variant,

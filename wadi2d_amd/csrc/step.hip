@@ -1291,16 +1291,25 @@ constexpr int EX_THREADS = 256;
 // one is non-zero. In a device-resident run (mem set) only the tiles of members that open a window before their
 // next step, with that window's (d0, d1, denom) from terms; in a protocol call (mem null) every tile, with the
 // given window. Each tile is one workgroup's; a tile's cells are all rated, listed or not (forcing.py:667-672 sets
-// every interior cell's rate).
+// every interior cell's rate). On the steps where no member's window is due (most), every workgroup leaves after
+// reading the M members' clocks.
 template <class R>
-__global__ __launch_bounds__(EX_THREADS) void excess_kernel(int nt, const Member* __restrict__ mem,
+__global__ __launch_bounds__(EX_THREADS) void excess_kernel(int nt, const Member* __restrict__ mem, int nmem,
                                                           const int* __restrict__ tile_member,
                                                           const double* __restrict__ terms, double d0a, double d1a,
                                                           double dna, const double* __restrict__ w,
                                                           const double* __restrict__ cn, R* __restrict__ src,
                                                           double* __restrict__ partial, int* __restrict__ flag) {
     __shared__ double red[EX_THREADS / 64];
-    __shared__ int any_nz;
+    __shared__ int any_nz, any_due;
+    if (mem) {
+        if (threadIdx.x == 0) any_due = 0;
+        __syncthreads();
+        for (int m = threadIdx.x; m < nmem; m += EX_THREADS)
+            if (window_due(mem[m])) any_due = 1;
+        __syncthreads();
+        if (!any_due) return;
+    }
     for (int t = 1 + blockIdx.x; t < nt; t += gridDim.x) {
         double d0 = d0a, d1 = d1a, dn = dna;
         if (mem) {
@@ -2714,11 +2723,11 @@ int w2d_source_excess(w2d_handle* H, double d0, double d1, double denom, double*
     const int nt = H->ntiles;
     if (H->prec == 8)
         excess_kernel<double><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
-            nt, nullptr, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (double*)H->src, H->ex_partial,
+            nt, nullptr, 0, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (double*)H->src, H->ex_partial,
             H->ex_flag);
     else
         excess_kernel<float><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
-            nt, nullptr, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (float*)H->src, H->ex_partial,
+            nt, nullptr, 0, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (float*)H->src, H->ex_partial,
             H->ex_flag);
     HIP_TRY(hipGetLastError());
     std::vector<double> part(nt, 0.0);
@@ -2922,7 +2931,7 @@ int enqueue_run_step(w2d_handle* H) {
     int s;
     if (H->excess) {
         excess_kernel<R><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
-            nt, H->run_mem, H->run_tile_member, H->run_terms, 0.0, 0.0, 1.0, H->src_weight, H->src_cn, (R*)H->src,
+            nt, H->run_mem, H->run_m, H->run_tile_member, H->run_terms, 0.0, 0.0, 1.0, H->src_weight, H->src_cn, (R*)H->src,
             H->run_src_partial, H->run_has_source);
         HIP_TRY(hipGetLastError());
     }
