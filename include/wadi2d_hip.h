@@ -127,6 +127,13 @@ int w2d_source_losses(w2d_handle* h, const int64_t* index, const double* w, cons
 /* One window's rates into every own cell's src; total = their FP64 sum (in tile order), and tiles[t] (ntiles, or
  * null) = 1 where a tile has a non-zero rate: the tiles set_source_cells keeps active (model.py:112-122). */
 int w2d_source_excess(w2d_handle* h, double d0, double d1, double denom, double* total, int32_t* tiles);
+/* Gridded rain (storm/gridded.py, hydrology/cell_rain.py:19-58): after w2d_source_losses, each listed own cell's
+ * pixel in [0, npix) (the cell reads the storm depths of that pixel instead of d0 and d1; unlisted cells take none).
+ * The form holds until w2d_source_losses or w2d_source_weights is called. */
+int w2d_source_pixels(w2d_handle* h, const int64_t* index, const int32_t* pixel, int64_t n, int32_t npix);
+/* w2d_source_excess for gridded rain: the npix pixels' cumulative depths at the window's start (d0) and end (d1). */
+int w2d_source_excess_pixels(w2d_handle* h, const double* d0, const double* d1, int32_t npix, double denom,
+                             double* total, int32_t* tiles);
 /* Kernel timing (no counterpart): with timing on, each step records HIP events on the handle's stream around its
  * face and cell kernels; timing_read waits, and returns and clears the summed milliseconds and the step count. */
 int w2d_timing(w2d_handle* h, int32_t on);
@@ -163,6 +170,9 @@ typedef struct w2d_member_state {
  * device as w2d_source_excess forms them, a window opens when its excess falls or the last one's did, and its kept
  * tiles are those with a non-zero rate (run.py:1719-1729). */
 int w2d_run_terms(w2d_handle* h, const double* terms, int64_t nwindows);
+/* With gridded rain, the pixels' depths of every window as well, before w2d_run_setup: nwindows rows of npix depths
+ * at the window's start then npix at its end, in the order of the terms. */
+int w2d_run_pixel_terms(w2d_handle* h, const double* depths, int64_t nwindows, int32_t npix);
 /* Set up a run from the handle's current state, active list and the stepper's idle counts (ntiles or null). */
 int w2d_run_setup(w2d_handle* h, const w2d_member* members, int32_t M, const double* scales, int64_t nscales,
                   const w2d_run_numerics* num, const int32_t* idle);

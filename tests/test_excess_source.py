@@ -25,12 +25,17 @@ from wadi2d_amd.layout import STRIDE2
 from wadi2d_amd.solver.model import Model
 from wadi2d_amd.solver.state import state_index
 
+from test_losses import frames_over
+
 pytestmark = pytest.mark.gpu
 
 FIELDS = ("h", "d", "u", "v", "mx", "my", "qx", "qy", "res", "src")
 
 
-def _valley(depth_grid: bool):
+def _valley(form: str):
+    """The crested valley with curve numbers (55-98, 100 on a strip, none on the upper valley) under one of three
+    storms: 'uniform' (50 mm over 30 min), 'depth_grid' (30-80 mm per cell), 'gridded' (seeded frames of 170 m
+    pixels, a gap and a missing frame value, area factor 0.9)."""
     tables = sloped_valley()
     ny, nx = tables.grid.shape
     east = Boundary("free", np.arange(1, ny - 1), np.full(ny - 2, nx - 2))
@@ -39,28 +44,42 @@ def _valley(depth_grid: bool):
     cn = rng.uniform(55.0, 98.0, (ny, nx)).round(1)
     cn[:, : nx // 4] = np.nan  # no losses on the upper valley
     cn[: ny // 3, nx // 2 : nx // 2 + 20] = 100.0
-    depths = rng.uniform(30.0, 80.0, (ny, nx)) if depth_grid else None
-    storm = uniform_storm(1.0 if depth_grid else 50.0, 1800.0)
-    rain = Rain(tables, bset, storm, curve_number=cn, depth_grid=depths)
+    if form == "gridded":
+        frames = frames_over(tables.grid)
+        rain = Rain(tables, bset, None, curve_number=cn, frames=frames, area_factor=0.9)
+    else:
+        depths = rng.uniform(30.0, 80.0, (ny, nx)) if form == "depth_grid" else None
+        storm = uniform_storm(1.0 if depths is not None else 50.0, 1800.0)
+        rain = Rain(tables, bset, storm, curve_number=cn, depth_grid=depths)
     return tables, bset, rain
 
 
-@pytest.mark.parametrize("depth_grid", [False, True])
-def test_one_window_of_device_rates_is_the_host_restatement_bit_for_bit(depth_grid):
-    tables, bset, rain = _valley(depth_grid)
+FORMS = ["uniform", "depth_grid", "gridded"]
+
+
+def _window(b, rain, t0, t1):
+    if rain.pixels is not None:
+        d = rain.pixel_terms(t0, t1)
+        return b.source_excess_pixels(d[0], d[1], rain.terms(t0, t1)[2])
+    return b.source_excess(*rain.terms(t0, t1))
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_one_window_of_device_rates_is_the_host_restatement_bit_for_bit(form):
+    tables, bset, rain = _valley(form)
     m = Model(tables, None, "hip-exact")
     m.set_depth(np.zeros(tables.grid.shape))
     Driver(m, bset, rain)  # w2d_source_losses
     b = m.backend
-    print(f"excess rates: depth grid {depth_grid}", flush=True)
+    print(f"excess rates: {form}", flush=True)
     jb, ib = np.nonzero(bset.mask)
     boundary = np.asarray(state_index(tables.grid, jb, ib), np.int64)
     assert boundary.size > 0
     seen_zero = seen_rain = False
-    for t0 in (0.0, 120.0, 300.0, 600.0, 1200.0, 1770.0, 1800.0, 2400.0):
+    for t0 in (0.0, 120.0, 300.0, 600.0, 1200.0, 1770.0, 1800.0, 2100.0, 2400.0, 4500.0):
         t1 = t0 + 30.0
         want, vol = rain.rate(t0, t1)
-        total, tiles = b.source_excess(*rain.terms(t0, t1))
+        total, tiles = _window(b, rain, t0, t1)
         src = b.get_field("src")
         assert np.array_equal(src[rain.state], want), t0
         assert not np.any(src[boundary]), t0  # the boundary cells take no rain
@@ -90,11 +109,11 @@ def _oracle_loop(tables, bset, rain, t_end):
     return m
 
 
-@pytest.mark.parametrize("depth_grid", [False, True])
-def test_the_host_loop_with_losses_is_the_oracle_bit_for_bit(depth_grid):
-    t_end = 2400.0
-    tables, bset, rain = _valley(depth_grid)
-    print(f"host loop vs oracle, excess form: depth grid {depth_grid}", flush=True)
+@pytest.mark.parametrize("form", FORMS)
+def test_the_host_loop_with_losses_is_the_oracle_bit_for_bit(form):
+    t_end = 2400.0 if form != "gridded" else 4800.0
+    tables, bset, rain = _valley(form)
+    print(f"host loop vs oracle, excess form: {form}", flush=True)
     ref = _oracle_loop(tables, bset, rain, t_end)
     m = Model(tables, None, "hip-exact")
     m.set_depth(np.zeros(tables.grid.shape))
@@ -113,15 +132,15 @@ def _plan(m, bset, rain, t_end):
     return MemberPlan(1, m.grid.ntiles, rain, bset, st.t, t_end, st.dt, st.nsteps, st.refresh_due, rain.state)
 
 
-@pytest.mark.parametrize("graph", [0, 16])
-def test_the_device_loop_with_losses_is_the_host_loop_bit_for_bit(graph):
-    t_end = 2400.0
-    tables, bset, rain = _valley(False)
+@pytest.mark.parametrize("graph, form", [(0, "uniform"), (16, "uniform"), (16, "gridded")])
+def test_the_device_loop_with_losses_is_the_host_loop_bit_for_bit(graph, form):
+    t_end = 2400.0 if form != "gridded" else 4800.0
+    tables, bset, rain = _valley(form)
     host = Model(tables, None, "hip-exact")
     host.set_depth(np.zeros(tables.grid.shape))
     drv = Driver(host, bset, rain)
     drv.run(t_end)
-    _, bset2, rain2 = _valley(False)
+    _, bset2, rain2 = _valley(form)
     dev = Model(tables, None, "hip-exact")
     dev.set_depth(np.zeros(tables.grid.shape))
     Driver(dev, bset2, rain2)

@@ -122,3 +122,74 @@ def test_curve_numbers_out_of_range_are_refused():
         Rain(tables, bset, uniform_storm(10.0, 600.0), curve_number=cn)
     with pytest.raises(ValueError):
         Rain(tables, bset, uniform_storm(10.0, 600.0), np.ones(tables.grid.shape), curve_number=cn * 0 + 70.0)
+
+
+# ---- gridded rain (storm/rain_frames.py, storm/gridded.py, hydrology/cell_rain.py)
+
+def test_a_pixel_series_is_linear_in_a_period_and_level_across_a_gap():
+    from wadi2d_amd.engine.gridded import PixelSeries, RainFrames
+
+    depths = np.array([[[10.0, 2.0]], [[5.0, np.nan]]])  # 2 frames, 1 row, 2 pixels; pixel 1 has no data in frame 2
+    frames = RainFrames(depths, np.array([0.0, 90.0]), np.array([60.0, 120.0]), 0.0, 100.0, 50.0, 100.0)
+    s = PixelSeries(frames, np.array([0, 1]))
+    for minutes, want in ((0.0, (0.0, 0.0)), (30.0, (5.0, 1.0)), (60.0, (10.0, 2.0)), (75.0, (10.0, 2.0)),
+                          (105.0, (12.5, 2.0)), (200.0, (15.0, 2.0))):  # fmt: skip
+        assert np.allclose(s.at(minutes), want, rtol=0, atol=1e-12), minutes
+    assert np.array_equal(frames.pixel_of(np.array([10.0, 60.0, 120.0, 10.0]), np.array([50.0, 50.0, 50.0, -5.0])),
+                          [0, 1, -1, -1])  # fmt: skip
+    assert np.array_equal(frames.totals(), [[15.0, 2.0]])
+    with pytest.raises(ValueError):
+        RainFrames(depths, np.array([0.0, 30.0]), np.array([60.0, 120.0]), 0.0, 100.0, 50.0, 100.0)  # overlap
+
+
+def frames_over(grid, pixel: float = 170.0, seed: int = 9, nan_frame: bool = True):
+    """Seeded rain frames covering the grid: 6 periods of 10 min with a gap (30-40 min), 0-15 mm each."""
+    from wadi2d_amd.engine.gridded import RainFrames
+
+    ny, nx = grid.shape
+    west, north = grid.x0 - 5.0, grid.y0 + 5.0
+    cols = int(np.ceil((nx * grid.dx + 10.0) / pixel))
+    rows = int(np.ceil((ny * grid.dx + 10.0) / pixel))
+    rng = np.random.default_rng(seed)
+    depths = rng.uniform(0.0, 15.0, (6, rows, cols))
+    if nan_frame:
+        depths[2, 0, 0] = np.nan  # a pixel missing in one frame counts as no rain there
+    starts = np.array([0.0, 10.0, 20.0, 40.0, 50.0, 60.0])
+    return RainFrames(depths, starts, starts + 10.0, west, north, pixel, pixel)
+
+
+def test_gridded_rain_gives_each_cell_its_pixel_through_its_losses():
+    tables, bset = _small()
+    grid = tables.grid
+    frames = frames_over(grid, pixel=37.0)
+    cn = np.full(grid.shape, 70.0)
+    cn[:, :10] = np.nan
+    rain = Rain(tables, bset, None, curve_number=cn, frames=frames, area_factor=0.9)
+    assert rain.excess_form and rain.pixels.used.size > 4
+    x, y = grid.xy_of(rain.jj, rain.ii)
+    pix = frames.pixel_of(x, y)
+    for t0 in (0.0, 600.0, 1500.0, 2700.0, 4200.0):
+        t1 = t0 + 30.0
+        got, vol = rain.rate(t0, t1)
+        # a cell's depth is its pixel's, times the area factor; no-CN cells pass it whole
+        for m in (0, 5, rain.state.size // 2, rain.state.size - 1):
+            col = np.nan_to_num(frames.depths.reshape(6, -1)[:, pix[m]])
+            knots = np.concatenate(([0.0], np.cumsum(col[:3]), np.sum(col[:3]) + np.cumsum(col[3:])))
+
+            def cum(t):  # the knots every 10 min, level from 30 to 40 min (the gap), times the area factor
+                return 0.9 * float(np.interp(t / 60.0, [0.0, 10.0, 20.0, 30.0, 40.0, 50.0, 60.0, 70.0],
+                                             np.insert(knots, 4, knots[3])))  # fmt: skip
+
+            cn_m = np.array([cn[rain.jj[m], rain.ii[m]]])
+            e0 = cell_excess(np.array([cum(t0)]), cn_m)[0]
+            e1 = cell_excess(np.array([cum(t1)]), cn_m)[0]
+            assert got[m] == pytest.approx((e1 - e0) / (1000.0 * 30.0), rel=1e-6, abs=1e-12), (t0, m)
+        assert vol >= 0.0
+    # a rained cell the grid does not cover is refused
+    small = frames_over(grid, pixel=37.0)
+    from wadi2d_amd.engine.gridded import RainFrames
+
+    cut = RainFrames(small.depths[:, 1:, :], small.starts_min, small.ends_min, small.west, small.north - 37.0,
+                     37.0, 37.0)  # fmt: skip
+    with pytest.raises(ValueError, match="does not cover"):
+        Rain(tables, bset, None, frames=cut)

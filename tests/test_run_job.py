@@ -346,9 +346,10 @@ def test_a_job_runs_end_to_end_bit_for_bit_the_oracle(job_folder):
     assert len(maps) >= 2 and (maps[0] / "depth.npy").exists()
 
 
-def _loss_job(folder: Path, holes: bool = False) -> dict:
+def _loss_job(folder: Path, holes: bool = False, gridded: bool = False) -> dict:
     """The test job with a curve number (80), a CN raster over the west half at cell resolution (a no-data patch
-    falling back to 80, unless ``holes`` drops the default) and a depth-grid storm rising from 20 to 70 mm eastward."""
+    falling back to 80, unless ``holes`` drops the default) and a depth-grid storm rising from 20 to 70 mm eastward,
+    or with ``gridded`` rain frames of 100 m pixels: four 5-minute periods of 0-12 mm with a 5-minute gap."""
     cn = np.full((NY, NX), -9999.0)
     cn[:, : NX // 2] = 60.0 + 30.0 * np.random.default_rng(4).random((NY, NX // 2))
     cn[10:20, 5:15] = -9999.0
@@ -358,6 +359,13 @@ def _loss_job(folder: Path, holes: bool = False) -> dict:
     d = job_dict()
     d["terrain"]["cn_raster"] = "cn.asc"
     d["storm"] = {"kind": "depth_grid", "depth_grid": "depth.asc", "duration_min": 20.0}
+    if gridded:
+        rng = np.random.default_rng(8)
+        cols, rows = NX * DX // 100 + 2, NY * DX // 100 + 2
+        np.savez(folder / "rain.npz", depths=rng.uniform(0.0, 12.0, (4, int(rows), int(cols))),
+                 starts_min=np.array([0.0, 5.0, 15.0, 20.0]), ends_min=np.array([5.0, 10.0, 20.0, 25.0]),
+                 west=X0 - 50.0, north=Y0 + NY * DX + 50.0, dx=100.0, dy=100.0)  # fmt: skip
+        d["storm"] = {"kind": "gridded", "rain_file": "rain.npz", "arf_factor": 0.8}
     if not holes:
         d["hydrology"] = {"curve_number": 80.0}
     (folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
@@ -385,6 +393,28 @@ def test_curve_numbers_and_a_depth_grid_reach_the_rain(job_folder):
     assert storm_curve(cfg)(600.0) == pytest.approx(0.5)  # the unit storm, half way
 
 
+def test_gridded_rain_reaches_the_rain_and_bad_files_are_refused(job_folder):
+    from wadi2d_amd.engine.forcing import boundary_set
+    from wadi2d_amd.run import boundary_list, build_grid, build_rain, storm_curve
+
+    _loss_job(job_folder, gridded=True)
+    cfg = jobmod.load_job(job_folder / "job.json")
+    assert Path(cfg.storm.rain_file).is_absolute() and storm_curve(cfg) is None
+    tables = build_grid(cfg)
+    bset = boundary_set(boundary_list(cfg, tables), tables)
+    rain = build_rain(cfg, tables, bset, storm_curve(cfg))
+    assert rain.pixels is not None and rain.pixels.factor == 0.8 and rain.pixels.used.size > 50
+    assert rain.rate(0.0, 300.0)[1] > 0.0 and rain.rate(600.0, 900.0)[1] == 0.0  # the gap
+    (job_folder / "rain.nc").write_bytes(b"CDF\x01")
+    for name, error in (("rain.nc", ValueError), ("absent.npz", FileNotFoundError)):
+        d = json.loads((job_folder / "job.json").read_text())
+        d["storm"]["rain_file"] = name
+        (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+        cfg = jobmod.load_job(job_folder / "job.json")
+        with pytest.raises(error):
+            build_rain(cfg, tables, bset, None)
+
+
 def test_a_loss_job_with_holes_or_bad_settings_is_refused(job_folder):
     from wadi2d_amd.engine.forcing import boundary_set
     from wadi2d_amd.run import boundary_list, build_grid, build_rain, storm_curve
@@ -406,14 +436,15 @@ def test_a_loss_job_with_holes_or_bad_settings_is_refused(job_folder):
 
 
 @pytest.mark.gpu
-def test_a_job_with_losses_and_a_depth_grid_runs_bit_for_bit_the_oracle(job_folder):
+@pytest.mark.parametrize("gridded", [False, True])
+def test_a_job_with_losses_and_a_depth_grid_or_gridded_rain_runs_bit_for_bit_the_oracle(job_folder, gridded):
     from wadi2d_amd.run import run_job
 
-    _loss_job(job_folder)
+    _loss_job(job_folder, gridded=gridded)
     cfg = jobmod.load_job(job_folder / "job.json")
     s = run_job(cfg, job_folder / "out", backend="hip-exact")
-    print(f"loss run: exit {s.exit_code} steps {s.steps} mass {s.balance.relative:.2e} rain {s.balance.rain:.1f}",
-          flush=True)
+    print(f"loss run (gridded {gridded}): exit {s.exit_code} steps {s.steps} mass {s.balance.relative:.2e} "
+          f"rain {s.balance.rain:.1f}", flush=True)
     assert s.exit_code == 0, s.message
     dmax, bal, steps = _oracle_run(cfg)
     got = np.load(job_folder / "out" / "results" / "maxima" / "depth.npy")

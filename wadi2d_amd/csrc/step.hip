@@ -1292,14 +1292,18 @@ constexpr int EX_THREADS = 256;
 // next step, with that window's (d0, d1, denom) from terms; in a protocol call (mem null) every tile, with the
 // given window. Each tile is one workgroup's; a tile's cells are all rated, listed or not (forcing.py:667-672 sets
 // every interior cell's rate). On the steps where no member's window is due (most), every workgroup leaves after
-// reading the M members' clocks.
+// reading the M members' clocks. With gridded rain (pix set) a cell's storm depths are its pixel's, pix[c] indexing
+// a row of npix depths at the window's start followed by npix at its end (pdep: the protocol call's row, or the
+// run's rows, one per window in the order of the terms); a cell with no pixel (-1) takes none.
 template <class R>
 __global__ __launch_bounds__(EX_THREADS) void excess_kernel(int nt, const Member* __restrict__ mem, int nmem,
                                                           const int* __restrict__ tile_member,
                                                           const double* __restrict__ terms, double d0a, double d1a,
                                                           double dna, const double* __restrict__ w,
-                                                          const double* __restrict__ cn, R* __restrict__ src,
-                                                          double* __restrict__ partial, int* __restrict__ flag) {
+                                                          const double* __restrict__ cn, const int* __restrict__ pix,
+                                                          const double* __restrict__ pdep, int npix,
+                                                          R* __restrict__ src, double* __restrict__ partial,
+                                                          int* __restrict__ flag) {
     __shared__ double red[EX_THREADS / 64];
     __shared__ int any_nz, any_due;
     if (mem) {
@@ -1312,14 +1316,17 @@ __global__ __launch_bounds__(EX_THREADS) void excess_kernel(int nt, const Member
     }
     for (int t = 1 + blockIdx.x; t < nt; t += gridDim.x) {
         double d0 = d0a, d1 = d1a, dn = dna;
+        const double* prow = pdep;
         if (mem) {
             const Member& b = mem[tile_member[t]];
             if (t < b.tile0 || t >= b.tile1 || !window_due(b)) continue;
             if (b.win < b.nwin) {
                 const double* q = terms + 3 * (b.scale_off + b.win);
                 d0 = q[0], d1 = q[1], dn = q[2];
+                if (pix) prow = pdep + (size_t)2 * npix * (size_t)(b.scale_off + b.win);
             } else {
                 d0 = 0.0, d1 = 0.0, dn = 1.0;
+                prow = nullptr;
             }
         }
         if (threadIdx.x == 0) any_nz = 0;
@@ -1329,7 +1336,13 @@ __global__ __launch_bounds__(EX_THREADS) void excess_kernel(int nt, const Member
 #pragma unroll
         for (int r = 0; r < TILE2 / EX_THREADS; ++r) {
             const size_t c = (size_t)t * TILE2 + r * EX_THREADS + threadIdx.x;
-            const R v = excess_rate<R>(w[c], cn[c], d0, d1, dn);
+            double c0 = d0, c1 = d1;
+            if (pix) {
+                const int q = pix[c];
+                c0 = q >= 0 && prow ? prow[q] : 0.0;
+                c1 = q >= 0 && prow ? prow[npix + q] : 0.0;
+            }
+            const R v = excess_rate<R>(w[c], cn[c], c0, c1, dn);
             src[c] = v;
             sum += (double)v;
             nz |= v != R(0);
@@ -1898,6 +1911,13 @@ struct w2d_handle {
     double* run_terms = nullptr;
     double* ex_partial = nullptr;  // per tile, a protocol call's sums and flags
     int* ex_flag = nullptr;
+    // gridded rain: each own cell's pixel (-1: none), a protocol call's pixel depths, a run's per-window rows
+    int* src_pix = nullptr;
+    int npix = 0;
+    double* pix_dep = nullptr;
+    size_t pix_dep_n = 0;
+    std::vector<double> run_pix_host;
+    double* run_pix_terms = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_parity = 0;  // the buffer parity the graph was captured at; it replays only at that parity
@@ -2714,21 +2734,30 @@ int w2d_source_losses(w2d_handle* H, const int64_t* index, const double* w, cons
     HIP_TRY(hipStreamSynchronize(H->stream));
     H->weight_host = own_w;
     H->excess = 1;
+    if (H->src_pix) {  // a new excess form starts without pixels: w2d_source_pixels sets them again
+        H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->src_pix), H->owned.end());
+        (void)hipFree(H->src_pix);
+        H->src_pix = nullptr;
+        H->npix = 0;
+    }
     return 0;
 }
 
-int w2d_source_excess(w2d_handle* H, double d0, double d1, double denom, double* total, int32_t* tiles) {
-    if (!H || !total || !(denom > 0.0)) return fail(1, "source_excess: bad arguments");
-    if (!H->excess) return fail(1, "source_excess: call source_losses first");
+}  // extern "C"
+
+namespace {
+
+int source_excess_any(w2d_handle* H, double d0, double d1, const int* pix, const double* pdep, int npix, double denom,
+                      double* total, int32_t* tiles) {
     const int nt = H->ntiles;
     if (H->prec == 8)
         excess_kernel<double><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
-            nt, nullptr, 0, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (double*)H->src, H->ex_partial,
-            H->ex_flag);
+            nt, nullptr, 0, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, pix, pdep, npix,
+            (double*)H->src, H->ex_partial, H->ex_flag);
     else
         excess_kernel<float><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
-            nt, nullptr, 0, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, (float*)H->src, H->ex_partial,
-            H->ex_flag);
+            nt, nullptr, 0, nullptr, nullptr, d0, d1, denom, H->src_weight, H->src_cn, pix, pdep, npix,
+            (float*)H->src, H->ex_partial, H->ex_flag);
     HIP_TRY(hipGetLastError());
     std::vector<double> part(nt, 0.0);
     std::vector<int32_t> flag(nt, 0);
@@ -2742,6 +2771,62 @@ int w2d_source_excess(w2d_handle* H, double d0, double d1, double denom, double*
         tiles[0] = 0;
         for (int t = 1; t < nt; ++t) tiles[t] = flag[t];
     }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int w2d_source_excess(w2d_handle* H, double d0, double d1, double denom, double* total, int32_t* tiles) {
+    if (!H || !total || !(denom > 0.0)) return fail(1, "source_excess: bad arguments");
+    if (!H->excess) return fail(1, "source_excess: call source_losses first");
+    if (H->src_pix) return fail(1, "source_excess: the source is gridded (w2d_source_excess_pixels)");
+    return source_excess_any(H, d0, d1, nullptr, nullptr, 0, denom, total, tiles);
+}
+
+int w2d_source_pixels(w2d_handle* H, const int64_t* index, const int32_t* pixel, int64_t n, int32_t npix) {
+    if (!H || n < 0 || npix < 1 || (n && (!index || !pixel))) return fail(1, "source_pixels: bad arguments");
+    if (!H->excess) return fail(1, "source_pixels: call source_losses first");
+    std::vector<int32_t> own(H->ncell, -1);
+    for (int64_t m = 0; m < n; ++m) {
+        const int64_t c = own_of(H, index[m]);
+        if (c < 0) return fail(1, "source_pixels: an index is not an own cell");
+        if (pixel[m] < 0 || pixel[m] >= npix) return fail(1, "source_pixels: a pixel is outside [0, npix)");
+        own[c] = pixel[m];
+    }
+    int s;
+    if (!H->src_pix && (s = dalloc(H, (void**)&H->src_pix, H->ncell * sizeof(int32_t)))) return s;
+    HIP_TRY(hipMemcpyAsync(H->src_pix, own.data(), H->ncell * sizeof(int32_t), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    H->npix = npix;
+    return 0;
+}
+
+int w2d_source_excess_pixels(w2d_handle* H, const double* d0, const double* d1, int32_t npix, double denom,
+                             double* total, int32_t* tiles) {
+    if (!H || !d0 || !d1 || !total || !(denom > 0.0)) return fail(1, "source_excess_pixels: bad arguments");
+    if (!H->excess || !H->src_pix) return fail(1, "source_excess_pixels: call source_losses and source_pixels first");
+    if (npix != H->npix) return fail(1, "source_excess_pixels: npix is not source_pixels' count");
+    int s;
+    if (H->pix_dep_n < 2 * (size_t)npix) {
+        if (H->pix_dep) {
+            H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->pix_dep), H->owned.end());
+            (void)hipFree(H->pix_dep);
+            H->pix_dep = nullptr;
+        }
+        if ((s = dalloc(H, (void**)&H->pix_dep, 2 * (size_t)npix * sizeof(double)))) return s;
+        H->pix_dep_n = 2 * (size_t)npix;
+    }
+    HIP_TRY(hipMemcpyAsync(H->pix_dep, d0, npix * sizeof(double), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipMemcpyAsync(H->pix_dep + npix, d1, npix * sizeof(double), hipMemcpyHostToDevice, H->stream));
+    return source_excess_any(H, 0.0, 0.0, H->src_pix, H->pix_dep, npix, denom, total, tiles);
+}
+
+int w2d_run_pixel_terms(w2d_handle* H, const double* depths, int64_t nwindows, int32_t npix) {
+    if (!H || nwindows < 0 || (nwindows && !depths)) return fail(1, "run_pixel_terms: bad arguments");
+    if (!H->src_pix || npix != H->npix) return fail(1, "run_pixel_terms: call source_pixels first, with this npix");
+    H->run_pix_host.assign(depths, depths + 2 * (size_t)npix * nwindows);
     return 0;
 }
 
@@ -2931,8 +3016,8 @@ int enqueue_run_step(w2d_handle* H) {
     int s;
     if (H->excess) {
         excess_kernel<R><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
-            nt, H->run_mem, H->run_m, H->run_tile_member, H->run_terms, 0.0, 0.0, 1.0, H->src_weight, H->src_cn, (R*)H->src,
-            H->run_src_partial, H->run_has_source);
+            nt, H->run_mem, H->run_m, H->run_tile_member, H->run_terms, 0.0, 0.0, 1.0, H->src_weight, H->src_cn,
+            H->src_pix, H->run_pix_terms, H->npix, (R*)H->src, H->run_src_partial, H->run_has_source);
         HIP_TRY(hipGetLastError());
     }
     if ((s = enqueue_pre<R, FAST>(H, 0))) return s;
@@ -3115,6 +3200,18 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         std::vector<double> tv = H->run_terms_host;
         if (tv.empty()) tv.assign(3, 0.0);
         if ((s = upload(H, &H->run_terms, tv.data(), tv.size()))) return s;
+        if (H->run_pix_terms) {
+            H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_pix_terms), H->owned.end());
+            (void)hipFree(H->run_pix_terms);
+            H->run_pix_terms = nullptr;
+        }
+        if (H->src_pix) {
+            if (H->run_pix_host.size() != 2 * (size_t)H->npix * (size_t)nscales)
+                return fail(1, "run_setup: gridded rain needs w2d_run_pixel_terms for every window first");
+            std::vector<double> pv = H->run_pix_host;
+            if (pv.empty()) pv.assign(2, 0.0);
+            if ((s = upload(H, &H->run_pix_terms, pv.data(), pv.size()))) return s;
+        }
     }
     if (num->maxima < 0) return fail(1, "run_setup: maxima is an interval in steps (0: none)");
     H->run_maxima = num->maxima;

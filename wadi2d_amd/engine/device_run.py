@@ -6,8 +6,9 @@ Each member keeps the reference's sequence: a window opens every ``source_every_
 forced wave refresh), a step is ``min(dt, window end - t, dt_max)``, the wave speed is reduced every ``dt_every``
 steps into the next ``dt``, the active list is checked every ``active_every`` steps. The window's source rates are
 its precomputed scale times each cell's weight, float(weight * scale), as ``Rain.rate`` forms them on the host; in
-the excess form (curve numbers, a depth grid) the device forms every cell's rate from the window's storm depths
-when the window is due, and the window's rain decides whether it opens.
+the excess form (curve numbers, a depth grid, gridded rain) the device forms every cell's rate from the window's
+storm depths (with gridded rain, its pixel's) when the window is due, and the window's rain decides whether it
+opens.
 """
 
 from __future__ import annotations
@@ -37,26 +38,26 @@ class MemberPlan:
     storage_index: np.ndarray | None = None
 
 
-def window_scales(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarray:
-    """The scale of every window of [t0, t_end], as the run loop opens them."""
+def window_bounds(t0: float, t_end: float, every: float) -> list[tuple[float, float]]:
+    """The windows of [t0, t_end] as the run loop opens them."""
     out = []
     t = t0
     while t < t_end - 1e-9:
         t_next = min(t + every, t_end)
-        out.append(rain.scale(t, t_next))
+        out.append((t, t_next))
         t = t_next
-    return np.asarray(out, np.float64)
+    return out
+
+
+def window_scales(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarray:
+    """The scale of every window of [t0, t_end] (the weight form's; the excess form's windows carry 0)."""
+    return np.asarray([0.0 if rain.excess_form else rain.scale(a, b) for a, b in window_bounds(t0, t_end, every)],
+                      np.float64)  # fmt: skip
 
 
 def window_terms(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarray:
     """(d0, d1, 1000 (t1 - t0)) of every window of [t0, t_end]: the excess form's windows as the device takes them."""
-    out = []
-    t = t0
-    while t < t_end - 1e-9:
-        t_next = min(t + every, t_end)
-        out.append(rain.terms(t, t_next))
-        t = t_next
-    return np.asarray(out, np.float64).reshape(-1, 3)
+    return np.asarray([rain.terms(a, b) for a, b in window_bounds(t0, t_end, every)], np.float64).reshape(-1, 3)
 
 
 class DeviceRun:
@@ -80,6 +81,12 @@ class DeviceRun:
             raise ValueError("a device run's members take one source form: all weighted, or all in the excess form")
         if excess == {True}:
             backend.run_terms(np.concatenate([window_terms(p.rain, p.t0, p.t_end, source_every_s) for p in plans]))
+            if plans[0].rain.pixels is not None:
+                if len(plans) != 1:
+                    raise ValueError("gridded rain runs one member per device run")
+                p = plans[0]
+                rows = [p.rain.pixel_terms(t0, t1) for t0, t1 in window_bounds(p.t0, p.t_end, source_every_s)]
+                backend.run_pixel_terms(np.asarray(rows).reshape(len(rows), 2, -1))
         rn = RunNumerics(num.cfl, num.dt_max, num.dt_min, num.wave_floor, num.dt_every, num.active_every, int(maxima))
         backend.run_setup(members, self.scales, rn, idle)
         self.state = backend.run_read()

@@ -58,6 +58,8 @@ class Driver:
             n = rain.state.size
             self.b.source_losses(rain.state, rain.depths if rain.depths is not None else np.ones(n),
                                  rain.cn if rain.cn is not None else np.full(n, np.nan))  # fmt: skip
+            if rain.pixels is not None:
+                self.b.source_pixels(rain.state, rain.pixels.column, rain.pixels.used.size)
             self.source_tiles = None
         else:
             weight = rain.weight if rain.weight is not None else np.ones(rain.state.size)
@@ -124,7 +126,11 @@ class Driver:
     def _open_excess(self, t: float, t_next: float) -> None:
         """The excess form (run.py:1719-1729): every cell's rate formed on the device; the window opens (kept tiles:
         those with a non-zero rate, model.py:112-122) when its rain is positive or the last window's was."""
-        total, tiles = self.b.source_excess(*self.rain.terms(t, t_next))
+        if self.rain.pixels is not None:
+            d = self.rain.pixel_terms(t, t_next)
+            total, tiles = self.b.source_excess_pixels(d[0], d[1], self.rain.terms(t, t_next)[2])
+        else:
+            total, tiles = self.b.source_excess(*self.rain.terms(t, t_next))
         rain = total * self.area * (t_next - t)
         if rain > 0.0 or not self.dry:
             m, st = self.model, self.model.stepper

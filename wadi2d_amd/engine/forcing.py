@@ -12,6 +12,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from wadi2d_amd.config import MM_PER_M
+from wadi2d_amd.engine.gridded import CellPixels, RainFrames
 from wadi2d_amd.engine.losses import cell_excess
 from wadi2d_amd.solver.backend import BoundaryCells
 from wadi2d_amd.solver.state import state_index
@@ -128,24 +129,32 @@ class Rain:
 
     * weight form: the curve times a per-cell weight (1 for a uniform storm). ``rate(t0, t1)`` is the window's mean
       rate per unit weight, ``scale``, times the weight, taken to float32 per cell.
-    * excess form (``curve_number`` or ``depth_grid`` given): the reference's own (forcing.py:644-678). A cell's
-      cumulative depth is the curve's times its depth factor (storm.depth_grid; the curve alone without one), its
-      cumulative excess that depth through the SCS loss of its curve number (NaN: none), and its rate the window's
-      difference of excess over 1000 (t1 - t0), taken to float32.
+    * excess form (``curve_number``, ``depth_grid`` or ``frames`` given): the reference's own (forcing.py:644-678).
+      A cell's cumulative depth is the curve's times its depth factor (storm.depth_grid; the curve alone without
+      one), or with gridded rain its pixel's (``frames``, times ``area_factor``), its cumulative excess that depth
+      through the SCS loss of its curve number (NaN: none), and its rate the window's difference of excess over
+      1000 (t1 - t0), taken to float32.
 
     Either way the window's rain volume is the FP64 sum of the float32 rates times the cell area and the window.
     """
 
     def __init__(self, tables, boundary: BoundarySet, cumulative_mm, weight: np.ndarray | None = None,
-                 curve_number: np.ndarray | None = None, depth_grid: np.ndarray | None = None) -> None:  # fmt: skip
+                 curve_number: np.ndarray | None = None, depth_grid: np.ndarray | None = None,
+                 frames: RainFrames | None = None, area_factor: float = 1.0) -> None:  # fmt: skip
         grid = tables.grid
         interior = grid.mask & ~boundary.mask
         self.jj, self.ii = np.nonzero(interior)
         self.state = np.asarray(state_index(grid, self.jj, self.ii), np.int64)
         self.area = grid.dx**2
         self.cumulative_mm = cumulative_mm
-        if weight is not None and (curve_number is not None or depth_grid is not None):
-            raise ValueError("a weighted storm takes no curve numbers or depth grid: give those as the excess form")
+        if weight is not None and (curve_number is not None or depth_grid is not None or frames is not None):
+            raise ValueError("a weighted storm takes no curve numbers, depth grid or frames: give the excess form")
+        if frames is not None and depth_grid is not None:
+            raise ValueError("gridded rain brings its own depths: no depth grid with it")
+        self.pixels = None
+        if frames is not None:
+            x, y = grid.xy_of(self.jj, self.ii)
+            self.pixels = CellPixels(frames, x, y, area_factor)
         self.weight = None if weight is None else np.asarray(weight, np.float64)[self.jj, self.ii]
         self.cn = None if curve_number is None else np.asarray(curve_number, np.float64)[self.jj, self.ii]
         self.depths = None if depth_grid is None else np.asarray(depth_grid, np.float64)[self.jj, self.ii]
@@ -156,20 +165,30 @@ class Rain:
 
     @property
     def excess_form(self) -> bool:
-        return self.cn is not None or self.depths is not None
+        return self.cn is not None or self.depths is not None or self.pixels is not None
 
     def scale(self, t0: float, t1: float) -> float:
         """The window's mean rate per unit weight, m/s (float64)."""
         return (float(self.cumulative_mm(t1)) - float(self.cumulative_mm(t0))) / (MM_PER_M * (t1 - t0))
 
     def terms(self, t0: float, t1: float) -> tuple[float, float, float]:
-        """The excess form's window as the device takes it: the storm depth at both ends, and 1000 (t1 - t0)."""
+        """The excess form's window as the device takes it: the storm depth at both ends (0 with gridded rain, whose
+        depths are the pixels'), and 1000 (t1 - t0)."""
+        if self.pixels is not None:
+            return 0.0, 0.0, MM_PER_M * (t1 - t0)
         return float(self.cumulative_mm(t0)), float(self.cumulative_mm(t1)), MM_PER_M * (t1 - t0)
+
+    def pixel_terms(self, t0: float, t1: float) -> np.ndarray:
+        """Gridded rain: the used pixels' cumulative depths at the window's start and end, (2, npix)."""
+        return np.stack((self.pixels.pixel_depths(t0), self.pixels.pixel_depths(t1)))
 
     def excess_mm(self, t: float) -> np.ndarray:
         """Cumulative excess per interior cell at time t (forcing.py:644-665)."""
-        depth = float(self.cumulative_mm(t))
-        rain = depth * self.depths if self.depths is not None else np.full(self.state.size, depth)
+        if self.pixels is not None:  # cell_rain.py:52-58
+            rain = np.array(self.pixels.cell_depths(t), dtype=np.float64)
+        else:
+            depth = float(self.cumulative_mm(t))
+            rain = depth * self.depths if self.depths is not None else np.full(self.state.size, depth)
         if self.cn is None:
             return rain
         return cell_excess(rain, self.cn)

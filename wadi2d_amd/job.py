@@ -82,6 +82,9 @@ class Storm(_Part):
     hyetograph_mm: list[float] | None = None
     depth_grid: str | None = Field(None, description="Storm depths per cell, mm (.asc or .npz), for kind "
                                    "'depth_grid': each cell takes its depth over duration_min (forcing.py:594-601)")
+    rain_file: str | None = Field(None, description="Rain frames for kind 'gridded' (.npz: depths mm per frame, "
+                                  "starts_min, ends_min, west, north, dx, dy); each cell reads the pixel under its "
+                                  "centre (storm/gridded.py, hydrology/cell_rain.py)")
 
     @model_validator(mode="after")
     def _complete(self) -> Storm:
@@ -90,6 +93,8 @@ class Storm(_Part):
             raise ValueError("a depth storm needs storm.depth_mm above zero; a raster of depths is kind 'depth_grid'")
         if self.kind == "depth_grid" and (self.depth_grid is None or self.depth_mm != 0.0):
             raise ValueError("a depth_grid storm needs storm.depth_grid, and takes its depths from it, not depth_mm")
+        if self.kind == "gridded" and self.rain_file is None:
+            raise ValueError("a gridded storm needs storm.rain_file")
         if self.kind == "hypothetical":
             t, d = self.time_percent, self.depth_percent
             if not t or not d or len(t) != len(d):
@@ -152,7 +157,7 @@ SWITCHED_OFF = (
     ("grid", "domain"), ("grid", "exclude_lumped"), ("grid", "exclude_subbasins"),
     ("terrain", "manning_raster"), ("terrain", "objects"), ("terrain", "roughness"),
     ("terrain", "fill_pits"),
-    ("storm", "rain_file"), ("storm", "gauges"), ("storm", "area_reduction"),
+    ("storm", "gauges"), ("storm", "area_reduction"),
     ("storm", "arf"), ("storm", "pattern"),
     ("numerics", "local_time_stepping"),
     ("mesh", "enabled"), ("strips", "enabled"), ("transmission", "enabled"),
@@ -232,7 +237,7 @@ def unsupported(cfg: RunConfig) -> list[str]:
     for name in TOP_LISTS_OFF:
         if getattr(cfg, name):
             out.append(name)
-    if cfg.storm.kind in ("gridded", "gauges"):
+    if cfg.storm.kind == "gauges":
         out.append(f"storm.kind '{cfg.storm.kind}'")
     out += [f"boundaries: kind 'flow' ({b.name or 'unnamed'})" for b in cfg.boundaries if b.kind == "flow"]
     return out
@@ -253,7 +258,7 @@ def ignored(cfg: RunConfig) -> list[str]:
 RUN_HYDROLOGY = ("curve_number",)
 """Keys of the hydrology section this path runs: the 2D cells' default curve number."""
 
-PATH_FIELDS = (("terrain", "dem"), ("terrain", "cn_raster"), ("storm", "depth_grid"))
+PATH_FIELDS = (("terrain", "dem"), ("terrain", "cn_raster"), ("storm", "depth_grid"), ("storm", "rain_file"))
 """Paths of a job this path reads; relative ones are relative to the job file (run.py:147-163, 209-220)."""
 
 

@@ -26,6 +26,7 @@ import numpy as np
 from wadi2d_amd.dem import grid_over, raster_at_cells, read_raster
 from wadi2d_amd.engine import forcing
 from wadi2d_amd.engine.driver import Balance, Driver
+from wadi2d_amd.engine.gridded import read_frames
 from wadi2d_amd.engine.lines import cells_along, inside, line_faces
 from wadi2d_amd.io import (
     SAVED_FIELDS,
@@ -89,6 +90,8 @@ def storm_curve(cfg: RunConfig):
         return forcing.uniform_storm(s.depth_mm * factor, s.duration_min * 60.0)
     if s.kind == "depth_grid":  # a unit depth, each cell's own from the grid (forcing.py:109)
         return forcing.uniform_storm(UNIT_DEPTH * factor, s.duration_min * 60.0)
+    if s.kind == "gridded":  # each cell reads its pixel's frames (build_rain): no one curve
+        return None
     if s.kind == "hypothetical":
         tp = np.concatenate(([0.0], np.asarray(s.time_percent, np.float64)))
         dp = np.concatenate(([0.0], np.asarray(s.depth_percent, np.float64)))
@@ -133,6 +136,9 @@ def build_rain(cfg: RunConfig, tables, bset, curve) -> forcing.Rain:
                                  "and nothing else does: extend the raster or set hydrology.curve_number")
         if np.any(np.isfinite(cn) & interior & ~((cn > 0.0) & (cn <= 100.0))):
             raise ValueError("terrain.cn_raster holds curve numbers outside (0, 100]")
+    frames = None
+    if cfg.storm.kind == "gridded":
+        frames = read_frames(cfg.storm.rain_file)
     depths = None
     if cfg.storm.kind == "depth_grid":  # forcing.py:594-601
         values = raster_at_cells(cfg.storm.depth_grid, grid)
@@ -141,7 +147,9 @@ def build_rain(cfg: RunConfig, tables, bset, curve) -> forcing.Rain:
             raise ValueError(f"storm.depth_grid has no depth on {holes} rained cells of {int(interior.sum())}")
         depths = np.nan_to_num(values)
     zero = lambda t: 0.0  # noqa: E731
-    return forcing.Rain(tables, bset, curve if curve is not None else zero, curve_number=cn, depth_grid=depths)
+    factor = cfg.storm.arf_factor if cfg.storm.arf_factor is not None else 1.0
+    return forcing.Rain(tables, bset, curve if curve is not None else zero, curve_number=cn, depth_grid=depths,
+                        frames=frames, area_factor=factor)  # fmt: skip
 
 
 def build_grid(cfg: RunConfig):
@@ -354,7 +362,8 @@ def _run(cfg: RunConfig, out: Path, backend: str | None, resume: bool, progress:
         elif saved.get("job_hash") != key:
             raise InvalidJob("the checkpoint belongs to another job: it cannot continue this one")
     (out / "config.resolved.json").write_text(cfg.model_dump_json(indent=2), encoding="utf-8")
-    if curve is None and not cfg.initial_levels and not any(b.kind == "stage" for b in cfg.boundaries):
+    quiet = curve is None and cfg.storm.kind != "gridded"
+    if quiet and not cfg.initial_levels and not any(b.kind == "stage" for b in cfg.boundaries):
         progress.emit("warn", message="no rain, no inflow and no stage boundary: nothing will happen")
     t0 = time.monotonic()
     tables = build_grid(cfg)

@@ -97,6 +97,9 @@ _SIGS = {
     "w2d_source_losses": [VP, VP, VP, VP, I64],
     "w2d_source_excess": [VP, D, D, D, ctypes.POINTER(D), VP],
     "w2d_run_terms": [VP, VP, I64],
+    "w2d_source_pixels": [VP, VP, VP, I64, I32],
+    "w2d_source_excess_pixels": [VP, VP, VP, I32, D, ctypes.POINTER(D), VP],
+    "w2d_run_pixel_terms": [VP, VP, I64, I32],
     "w2d_device_count": [ctypes.POINTER(I32)],
     "w2d_build_tables_device": [VP, I64, I64, I64, I64, D, I32, I32, D, VP, I32, VP, VP, I32,
                                 VP, VP, VP, VP, VP, VP, VP, VP],  # fmt: skip
@@ -346,6 +349,27 @@ class HipBackend:
         check(api().w2d_source_excess(self.h, float(d0), float(d1), float(denom), ctypes.byref(total),
                                       tiles.ctypes.data))  # fmt: skip
         return float(total.value), tiles.astype(bool)
+
+    def source_pixels(self, index: np.ndarray, pixel: np.ndarray, npix: int) -> None:
+        """Gridded rain: each listed own cell's pixel, a column of the per-window pixel depths."""
+        idx, px = _c(index, np.int64), _c(pixel, np.int32)
+        if idx.size != px.size:
+            raise ValueError("source_pixels: index and pixel differ in length")
+        check(api().w2d_source_pixels(self.h, idx.ctypes.data, px.ctypes.data, int(idx.size), int(npix)))
+
+    def source_excess_pixels(self, d0: np.ndarray, d1: np.ndarray, denom: float) -> tuple[float, np.ndarray]:
+        """One window of gridded rain: the pixels' depths at its ends; the rates' FP64 sum and the kept tiles."""
+        a, b2 = _c(d0, np.float64), _c(d1, np.float64)
+        total = D(0.0)
+        tiles = np.zeros(self.grid.ntiles, np.int32)
+        check(api().w2d_source_excess_pixels(self.h, a.ctypes.data, b2.ctypes.data, int(a.size), float(denom),
+                                             ctypes.byref(total), tiles.ctypes.data))  # fmt: skip
+        return float(total.value), tiles.astype(bool)
+
+    def run_pixel_terms(self, depths: np.ndarray) -> None:
+        """Gridded rain in a device run: (nwindows, 2, npix) pixel depths at each window's start and end."""
+        d = _c(np.asarray(depths), np.float64)
+        check(api().w2d_run_pixel_terms(self.h, d.ctypes.data if d.size else None, int(d.shape[0]), int(d.shape[2])))
 
     def run_terms(self, terms: np.ndarray) -> None:
         """(d0, d1, 1000 (t1 - t0)) of every window of a device run, as (nwindows, 3)."""
