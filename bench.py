@@ -72,7 +72,7 @@ def main() -> int:
                     help="step: build the sub-grid tables on the host (OpenMP) or on the GPU")
     ap.add_argument("--losses", choices=("off", "on"), default="off",
                     help="step: the storm as a depth grid with SCS curve-number losses (CN 60-95), rates formed on "
-                         "the device each window")  # fmt: skip
+                         "the device each window; ensemble: every member with its own curve numbers")  # fmt: skip
     ap.add_argument("--members", type=int, default=64, help="ensemble members")
     ap.add_argument("--serial-members", type=int, default=-1, help="members the serial baseline runs (-1: all)")
     ap.add_argument("--reference", choices=("on", "off"), default="on", help="the FP64 batched run (ensemble)")
@@ -240,6 +240,9 @@ def ensemble(args) -> int:
     t0 = time.time()
     specs = draw_members(args.members)
     ens = Ensemble.build(specs)
+    if args.losses == "on":  # each member its own seeded curve numbers, 60-95
+        for spec, sc in zip(specs, ens.scenarios, strict=True):
+            sc.curve_number = 60.0 + 35.0 * np.random.default_rng(spec.seed + 1).random(sc.tables.grid.shape)
     log(f"[bench] {len(specs)} members, {ens.cells} cells, {ens.tables.grid.ntiles - 1} tiles: built in "
         f"{time.time() - t0:.1f} s")
     ref = None
@@ -273,7 +276,8 @@ def ensemble(args) -> int:
         "data": "synthetic (seeded members, tables and storms built in-process)",
         "config": {
             "workload": f"{len(specs)} members, sides from 128..1024 at 20 m ({ens.cells} cells), own seeded ground "
-                        "each, NRCS Type II storms of 20-120 mm over 1-6 h times an ARF of 0.6-1.0, 3 h each",
+                        "each, NRCS Type II storms of 20-120 mm over 1-6 h times an ARF of 0.6-1.0, 3 h each"
+                        + (", SCS losses (CN 60-95 per cell) formed on the device" if args.losses == "on" else ""),
             "launch": list(got.launch) if got.launch else None,
             "maxima_every": args.maxima_every,
             "plain_member_hours_per_hour": plain_rate,

@@ -91,6 +91,27 @@ def test_batched_exact_equals_each_member_run_alone():
 
 
 @pytest.mark.gpu
+def test_batched_exact_with_curve_number_losses_equals_each_member_run_alone():
+    """Members with SCS losses (two of three; the third keeps its rain whole) in one launch: each member's rates
+    formed on the device in the excess form, bit for bit what it gets alone the plain way."""
+    ens = _small(1800.0)
+    for m in (0, 2):
+        shape = ens.scenarios[m].tables.grid.shape
+        cn = 55.0 + 40.0 * np.random.default_rng(40 + m).random(shape)
+        cn[:, : shape[1] // 3] = np.nan
+        ens.scenarios[m].curve_number = cn
+    assert ens.losses and np.isnan(ens.member_cn(1)).all()
+    got = run_batched(ens, "hip-exact", graph_steps=16, chunk=128, tune=False)
+    alone = run_serial(ens, "hip-exact")  # member 1 alone takes the weight form: the same rates, bit for bit
+    print("losses: steps batched", got.steps, "alone", alone.steps, flush=True)
+    assert got.steps == alone.steps
+    for m in range(3):
+        assert np.array_equal(got.max_depth[m], alone.max_depth[m]), m
+        assert got.mass_error[m] == pytest.approx(alone.mass_error[m], abs=1e-12)
+        assert (got.max_depth[m] > 0.01).sum() > 100, m
+
+
+@pytest.mark.gpu
 def test_batched_fast_float32_meets_the_fp64_oracle():
     ens = _small(900.0)
     got = run_batched(ens, "hip", graph_steps=16, chunk=128)

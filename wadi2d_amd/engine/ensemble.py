@@ -109,16 +109,26 @@ def stack_tables(members: list[GridTables]) -> tuple[GridTables, list[Slot]]:
 
 class MemberRain:
     """A member's storm (cumulative depth in mm against time in s) over its interior cells (state indices of the
-    mosaic): the window scale the device forms rates from, as ``Rain.scale`` (the weight form)."""
+    mosaic): the window scale the device forms rates from, as ``Rain.scale`` (the weight form); with curve numbers
+    (``cn`` per interior cell, NaN: none) the window's storm depths, as ``Rain.terms`` (the excess form)."""
 
-    excess_form = False
+    pixels = None
+    depths = None
 
-    def __init__(self, storm, state: np.ndarray) -> None:
+    def __init__(self, storm, state: np.ndarray, cn: np.ndarray | None = None) -> None:
         self.storm = storm
         self.state = state
+        self.cn = cn
+
+    @property
+    def excess_form(self) -> bool:
+        return self.cn is not None
 
     def scale(self, t0: float, t1: float) -> float:
         return (float(self.storm(t1)) - float(self.storm(t0))) / (1000.0 * (t1 - t0))
+
+    def terms(self, t0: float, t1: float) -> tuple[float, float, float]:
+        return float(self.storm(t0)), float(self.storm(t1)), 1000.0 * (t1 - t0)
 
 
 @dataclass
@@ -129,6 +139,7 @@ class Ensemble:
     slots: list[Slot]
     boundaries: list[list[Boundary]]
     interior: list[np.ndarray] = field(default_factory=list)  # per member, mosaic state indices
+    interior_ji: list[tuple[np.ndarray, np.ndarray]] = field(default_factory=list)  # the same on the member's grid
 
     @classmethod
     def build(cls, specs: list[MemberSpec], dx: float = 20.0, k: int = 4, duration_s: float = 3 * 3600.0,
@@ -158,11 +169,23 @@ class Ensemble:
             part[sl.row0 : sl.row0 + ny, :nx] = interior[sl.row0 : sl.row0 + ny, :nx]
             jj, ii = np.nonzero(part)
             ens.interior.append(np.asarray(state_index(tables.grid, jj, ii), np.int64))
+            ens.interior_ji.append((jj - sl.row0, ii))
         ens.bset = bset
         return ens
 
     def storm(self, m: int):
         return self.scenarios[m].cumulative_mm
+
+    @property
+    def losses(self) -> bool:
+        """Whether any member has curve numbers: then every member's rain is in the excess form."""
+        return any(getattr(sc, "curve_number", None) is not None for sc in self.scenarios)
+
+    def member_cn(self, m: int) -> np.ndarray:
+        """Member m's curve numbers at its interior cells, in the order of ``interior[m]`` (NaN without)."""
+        jj, ii = self.interior_ji[m]
+        cn = getattr(self.scenarios[m], "curve_number", None)
+        return np.full(jj.size, np.nan) if cn is None else np.asarray(cn, np.float64)[jj, ii]
 
     @property
     def cells(self) -> int:
@@ -205,7 +228,12 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
     b = model.backend
     model.set_boundary(ens.bset.cells)
     state_all = np.concatenate(ens.interior)
-    b.source_weights(state_all, np.ones(state_all.size))
+    losses = ens.losses
+    if losses:
+        b.source_losses(state_all, np.ones(state_all.size), np.concatenate([ens.member_cn(m) for m in
+                                                                            range(len(ens.slots))]))  # fmt: skip
+    else:
+        b.source_weights(state_all, np.ones(state_all.size))
     member_of = ens.tile_member()
     faces = np.concatenate((ens.bset.xfaces, ens.bset.yfaces)).astype(np.int64)
     axis = np.concatenate((np.zeros(ens.bset.xfaces.size), np.ones(ens.bset.yfaces.size))).astype(np.int32)
@@ -217,7 +245,7 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
     st = model.stepper
     plans = []
     for m, sl in enumerate(ens.slots):
-        rain = MemberRain(ens.storm(m), ens.interior[m])
+        rain = MemberRain(ens.storm(m), ens.interior[m], ens.member_cn(m) if losses else None)
         plans.append(MemberPlan(sl.tile0, sl.tile1, rain, None, 0.0, ens.scenarios[m].duration_s, st.dt, 0, True,
                                 ens.interior[m]))  # fmt: skip
     if prepare is not None:
@@ -271,7 +299,7 @@ def run_serial(ens: Ensemble, backend: str = "hip", members: list[int] | None = 
         sc = ens.scenarios[m]
         tables = sc.tables
         bset = boundary_set(sc.boundaries, tables)
-        rain = Rain(tables, bset, ens.storm(m))
+        rain = Rain(tables, bset, ens.storm(m), curve_number=getattr(sc, "curve_number", None))
         t0 = time.perf_counter()
         model = Model(tables, sc.numerics, backend)
         model.set_depth(np.zeros(tables.grid.shape))

@@ -51,6 +51,7 @@ class Scenario:
     duration_s: float
     numerics: Numerics
     tables: object = None
+    curve_number: np.ndarray | None = None  # SCS curve numbers on the padded grid (NaN: no losses), or none at all
 
     def build(self, workers: int | None = None):
         if self.tables is None:
