@@ -71,8 +71,8 @@ def main() -> int:
         model.save(args.out)
     held = list(range(5000, 5000 + args.held_out_members))
     hcol, hpairs = fine_runs(held, args.n, args.duration, chunk=64, log=log, storm_seed=1)
-    held = {"plain": score(None, hcol, hpairs), "learned": score(model, hcol, hpairs)}
-    print(json.dumps({"samples": int(x.shape[0]), "history": history, "calibration": scores, "held_out": held,
+    held_scores = {"plain": score(None, hcol, hpairs), "learned": score(model, hcol, hpairs)}
+    print(json.dumps({"samples": int(x.shape[0]), "history": history, "calibration": scores, "held_out": held_scores,
                       "chosen": {"conveyance": model.conveyance, "rise_scale": model.rise_scale},
                       "wall_s": time.time() - t0}), flush=True)  # fmt: skip
     return 0
