@@ -71,6 +71,10 @@ int w2d_set_field(w2d_handle* h, int32_t field, const void* host, int64_t n);
 int w2d_get_field(w2d_handle* h, int32_t field, void* host, int64_t n);
 /* Model.read_cells (model.py:139-143): entries at own-cell state indices. */
 int w2d_read_cells(w2d_handle* h, int32_t field, const int64_t* index, int64_t n, void* out);
+/* The writes the reference's callers make through its arrays (momentum.py:49-54 FaceKick.apply into mx; tests into u,
+ * v, h): values (the handle's type) into own cells at state indices, under set_field's rules (H both buffers, the
+ * momentum stored first and written to both buffers); an index given twice keeps its last value. */
+int w2d_write_cells(w2d_handle* h, int32_t field, const int64_t* index, int64_t n, const void* values);
 
 /* set_active (gpu/backend.py:116-119). */
 int w2d_set_active(w2d_handle* h, const int32_t* tiles, int32_t n);

@@ -160,3 +160,33 @@ def test_a_momentum_or_depth_set_between_steps_is_taken_as_given():
         m.backend.set_field("h", to_state(m.grid, h, np.float32))
         m.advance(12.0)
     _compare(hip, ora, 0.0, "exact f32 t=12 after a depth change")
+
+
+@pytest.mark.gpu
+def test_cells_written_between_steps_are_what_set_field_would_give():
+    """w2d_write_cells is the reference callers' writes through the backend's arrays (FaceKick.apply's
+    b.mx[faces], momentum.py:49-54; tests' b.u[idx]): a kick on some faces, a velocity and a depth written cell by
+    cell, then steps, must leave every field bit for bit the oracle's after the same whole-array set_field."""
+    from wadi2d_amd.solver.state import state_index, to_state
+
+    tables, h0 = rough_dam_break()
+    num = Numerics(dt_every=1, dt_max=0.2)
+    hip, ora = _pair(tables, num, "hip-exact", 4, h0=h0)
+    for m in (hip, ora):
+        m.advance(4.0)
+    rng = np.random.default_rng(21)
+    jj, ii = np.nonzero(tables.grid.mask)
+    pick = rng.choice(jj.size, 400, replace=False)
+    j, i = jj[pick], ii[pick]
+    idx = np.asarray(state_index(tables.grid, j, i), np.int64)
+    for name, scale in (("mx", 1.5), ("u", 0.5), ("h", 1.25)):
+        whole = ora.field(name).astype(np.float32)
+        whole[j, i] *= np.float32(scale)
+        ora.backend.set_field(name, to_state(ora.grid, whole, np.float32))
+        values = hip.field(name).astype(np.float32)[j, i] * np.float32(scale)
+        # the first write of each cell is overwritten by the second: the last one counts
+        hip.backend.write_cells(name, np.concatenate((idx, idx)), np.concatenate((values * 0, values)))
+        assert np.array_equal(hip.field(name), whole), name
+    for m in (hip, ora):
+        m.advance(9.0)
+    _compare(hip, ora, 0.0, "exact f32 t=9 after cells written")

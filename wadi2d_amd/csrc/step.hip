@@ -1180,6 +1180,13 @@ __global__ __launch_bounds__(NT) void maxima_kernel(const int* __restrict__ acti
     }
 }
 
+// Cells written from the host (w2d_write_cells): value q into own cell at[q].
+template <class T>
+__global__ void scatter_kernel(int64_t n, const int64_t* __restrict__ at, const T* __restrict__ v, T* __restrict__ dst) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        dst[at[q]] = v[q];
+}
+
 // The window's source rate of every cell from its weight: float(weight * scale), the product in double.
 template <class R>
 __global__ void scale_source_kernel(size_t n, const double* __restrict__ w, double s, R* __restrict__ src) {
@@ -2370,6 +2377,58 @@ int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, vo
         HIP_TRY(hipMemcpyAsync((char*)out + m * es, (char*)src + c * es, es, hipMemcpyDeviceToHost, H->stream));
     }
     HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_write_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, const void* values) {
+    if (!H || n < 0 || (n && (!index || !values))) return fail(1, "write_cells: bad arguments");
+    void* dst = field_ptr(H, f);
+    if (!dst) return fail(1, "write_cells: the device keeps no copy of this field");
+    if (n == 0) return 0;
+    const size_t es = esize(H);
+    // own cells of real tiles; an index given twice keeps its last value, as an assignment through it would
+    std::vector<std::pair<int64_t, int64_t>> cm(n);  // (own cell, position in the call)
+    for (int64_t m = 0; m < n; ++m) {
+        const int64_t c = own_of(H, index[m]);
+        if (c < (int64_t)TILE2) return fail(1, "write_cells: an index is not an own cell of a tile");
+        cm[m] = {c, m};
+    }
+    std::sort(cm.begin(), cm.end());
+    std::vector<int64_t> at;
+    std::vector<char> val;
+    for (size_t q = 0; q < cm.size(); ++q) {
+        if (q + 1 < cm.size() && cm[q + 1].first == cm[q].first) continue;  // a later write of the same cell
+        const int64_t m = cm[q].second;
+        at.push_back(cm[q].first);
+        val.insert(val.end(), (const char*)values + m * es, (const char*)values + (m + 1) * es);
+    }
+    int s;
+    if (momentum_field(f) && (s = materialize(H))) return s;
+    const int64_t k = (int64_t)at.size();
+    void* buf = nullptr;
+    HIP_TRY(hipMalloc(&buf, k * (sizeof(int64_t) + es)));
+    int64_t* d_at = (int64_t*)buf;
+    void* d_val = (char*)buf + k * sizeof(int64_t);
+    std::vector<void*> targets{dst};
+    if (f == W2D_H) targets.push_back(H->h[1 - H->hcur]);
+    if (f == W2D_MX || f == W2D_MY || f == W2D_MX1 || f == W2D_MY1) {
+        const int q = (f == W2D_MX || f == W2D_MX1) ? 2 : 3;
+        targets = {H->uv[q][0], H->uv[q][1]};
+    }
+    hipError_t e = hipMemcpyAsync(d_at, at.data(), k * sizeof(int64_t), hipMemcpyHostToDevice, H->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_val, val.data(), k * es, hipMemcpyHostToDevice, H->stream);
+    const unsigned blocks = (unsigned)std::min<int64_t>((k + 255) / 256, 1024);
+    for (void* t : targets) {
+        if (e != hipSuccess) break;
+        if (es == 8)
+            scatter_kernel<double><<<blocks, 256, 0, H->stream>>>(k, d_at, (const double*)d_val, (double*)t);
+        else
+            scatter_kernel<float><<<blocks, 256, 0, H->stream>>>(k, d_at, (const float*)d_val, (float*)t);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(H->stream);
+    (void)hipFree(buf);
+    if (e != hipSuccess) return fail(3, hipGetErrorString(e));
     return 0;
 }
 

@@ -57,6 +57,7 @@ _SIGS = {
     "w2d_set_field": [VP, I32, VP, I64],
     "w2d_get_field": [VP, I32, VP, I64],
     "w2d_read_cells": [VP, I32, VP, I64, VP],
+    "w2d_write_cells": [VP, I32, VP, I64, VP],
     "w2d_set_active": [VP, VP, I32],
     "w2d_set_boundary": [VP, VP, VP, VP, VP, VP, I32],
     "w2d_step": [VP, D, I32, I32],
@@ -219,6 +220,12 @@ class HipBackend:
         out = np.zeros(self.n, self.dtype)
         check(api().w2d_get_field(self.h, FIELDS.index(name), out.ctypes.data, self.n))
         return out
+
+    def write_cells(self, name: str, index: np.ndarray, values) -> None:
+        """Values into own cells at state indices (the writes callers make through the reference's arrays)."""
+        idx = _c(index, np.int64)
+        val = _c(np.broadcast_to(np.asarray(values, self.dtype), idx.shape), self.dtype)
+        check(api().w2d_write_cells(self.h, FIELDS.index(name), idx.ctypes.data, int(idx.size), val.ctypes.data))
 
     def read_cells(self, name: str, index: np.ndarray) -> np.ndarray:
         idx = _c(index, np.int64)
