@@ -10,7 +10,6 @@ float32 ulp of the host's and almost all must be equal.
 
 from __future__ import annotations
 
-from pathlib import Path
 
 import numpy as np
 import pytest
