@@ -1703,18 +1703,24 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         }
     }
     if (changed && threadIdx.x == 0) g->changed = 1;
-    // the last workgroup to finish gathers the members' lists (every other one has published its own first)
+    // the last workgroup to finish gathers the members' lists (every other one has published its own first); a
+    // lone workgroup is the last without the handshake (its device-scope fences and atomic)
     __shared__ int is_last;
+    const bool alone = gridDim.x == 1;
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
-        is_last = atomicAdd(&g->pre_done, 1) == (int)gridDim.x - 1;
+        if (alone) {
+            is_last = 1;
+        } else {
+            __threadfence();
+            is_last = atomicAdd(&g->pre_done, 1) == (int)gridDim.x - 1;
+        }
     }
     __syncthreads();
     if (!is_last) return;
-    __threadfence();
+    if (!alone) __threadfence();
     gather_lists(mem, (int)gridDim.x, g, L.seg, L.active, wave_cnt);
-    if (threadIdx.x == 0) g->pre_done = 0;
+    if (threadIdx.x == 0 && !alone) g->pre_done = 0;
 }
 
 // After a step, one workgroup per member: its boundary cells (cells.cu:95-111) and FP64 tally
