@@ -847,7 +847,8 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     __shared__ __align__(16) R s_h[LSIZE];
     __shared__ __align__(16) R s_qx[LSIZE];
     __shared__ __align__(16) R s_qy[LSIZE];
-    __shared__ __align__(16) R s_r[LSIZE];
+    constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32, compact (7 workgroups per CU fit the LDS)
+    __shared__ __align__(16) R s_r[RW * RW];
     __shared__ int nb[9];
     __shared__ int any_wet;
     __shared__ double red_src[NTH / 64];
@@ -879,14 +880,13 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     const R dx = R(A.P.dx);
     const R area = dx * dx;
     const R dt = dt_run;
-    constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32
     for (int p = threadIdx.x; p < RW * RW; p += NTH) {
         const int jj = p / RW, ii = p - (p / RW) * RW;
         const int L = lds_of(jj - 1, ii - 1);
         const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + LS], R(0)) +
                       pmax(-s_qy[L], R(0));
         const R have = pmax(s_h[L], R(0)) * area;
-        s_r[L] = out * dt > have ? dv<FAST>(have, out * dt) : R(1);
+        s_r[p] = out * dt > have ? dv<FAST>(have, out * dt) : R(1);
     }
     __syncthreads();
     const int i = threadIdx.x & (TILE - 1);
@@ -920,12 +920,13 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     for (int r = 0; r < RT; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
+        const int Q = (j + 1) * RW + (i + 1);  // the cell in s_r
         const unsigned c = cs[r];
-        const R rc = s_r[L];
-        const R qw = scaled(s_qx[L], s_r[L - 1], rc);
-        const R qe = scaled(s_qx[L + 1], rc, s_r[L + 1]);
-        const R qn = scaled(s_qy[L], s_r[L - LS], rc);
-        const R qs = scaled(s_qy[L + LS], rc, s_r[L + LS]);
+        const R rc = s_r[Q];
+        const R qw = scaled(s_qx[L], s_r[Q - 1], rc);
+        const R qe = scaled(s_qx[L + 1], rc, s_r[Q + 1]);
+        const R qn = scaled(s_qy[L], s_r[Q - RW], rc);
+        const R qs = scaled(s_qy[L + LS], rc, s_r[Q + RW]);
         qw_v[r] = qw;
         qn_v[r] = qn;
         const R h0 = s_h[L];
