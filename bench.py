@@ -203,8 +203,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic (seeded ground, tables and storm built in-process)",
         "config": {
-            "workload": f"plain FP32 step, {args.size}x{args.size} cells at 30 m, 12-row sub-grid tables, hot start "
-                        f"at t={hot:.0f} s of a Gaussian 80 mm/h storm, four free outflows"
+            "workload": f"plain FP32 step, {args.size}x{args.size} cells at 30 m, 12-row sub-grid tables, "
+                        + (f"hot start at t={hot:.0f} s" if hot > 0 else "cold start (t=0, dry ground)")
+                        + " of a Gaussian 80 mm/h storm, four free outflows"
                         + (", the storm as a depth grid with SCS losses (CN 60-95) formed on the device"
                            if args.losses == "on" else ""),
             "cells": domain_cells,
