@@ -157,6 +157,7 @@ typedef struct w2d_member {
     int64_t scale_offset;
     int64_t nsteps0;
     int32_t refresh_due;
+    int32_t dry;  /* the last window's rain was nothing (run.py:1708, 1729): a resumed run's state */
 } w2d_member;
 /* maxima: the running maxima are sampled after every step whose count is a multiple of it (0: not sampled), as
  * the run loop's maxima_every (engine/maxima.py:103-146, config/schema.py:1001). */
@@ -167,7 +168,8 @@ typedef struct w2d_run_numerics {
 typedef struct w2d_member_state {
     double t, dt, rain;
     int64_t nsteps;
-    int32_t done, failed, refresh_due;
+    int32_t done, failed, refresh_due, dry;
+    double dt_min; /* the smallest step length dt held after any step of this run (the run summary's dt_min) */
 } w2d_member_state;
 /* With the excess source (w2d_source_losses), a device-resident run takes (d0, d1, denom) per window, in the order of
  * the scales (3 nwindows doubles, nwindows = nscales), before w2d_run_setup: its windows' rates are formed on the
@@ -186,6 +188,10 @@ int w2d_run_steps(w2d_handle* h, int32_t nsteps, int32_t graph_steps);
 /* Wait, and read every member's clock, step, count, rain volume and flags (failed: 1 non-finite wave, 2 step below
  * the minimum). */
 int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
+/* Hold each member once its clock t >= t_stop[m] (no counterpart: the host reads outputs at the step that reaches
+ * an output time, as the host loop's step hook does); +inf lets it run on. Steps enqueued while a member is held
+ * change nothing of it. */
+int w2d_run_pause(w2d_handle* h, const double* t_stop, int32_t M);
 int w2d_run_idle(w2d_handle* h, int32_t* out);
 /* The active list as the device last left it (out may be null to ask for the count). */
 int w2d_active_tiles(w2d_handle* h, int32_t* out, int32_t* n);

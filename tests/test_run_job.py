@@ -480,6 +480,61 @@ def test_a_stopped_and_resumed_run_is_bit_for_bit_the_unbroken_one(job_folder):
     assert s.exit_code == 2
 
 
+def _same_outputs(a: Path, b: Path) -> None:
+    for name in ("depth", "mean_depth", "speed", "hazard", "arrival"):
+        x = np.load(a / "results" / "maxima" / f"{name}.npy")
+        y = np.load(b / "results" / "maxima" / f"{name}.npy")
+        assert np.array_equal(x, y, equal_nan=True), name
+    for f in ("lines_gap.csv", "points_p1.csv"):
+        assert (a / "series" / f).read_text() == (b / "series" / f).read_text(), f
+    maps_a = sorted(p.name for p in (a / "results" / "maps").iterdir())
+    assert maps_a == sorted(p.name for p in (b / "results" / "maps").iterdir())
+    for m in maps_a:
+        for q in ("depth", "mean_depth", "qx"):
+            x = np.load(a / "results" / "maps" / m / f"{q}.npy")
+            assert np.array_equal(x, np.load(b / "results" / "maps" / m / f"{q}.npy")), (m, q)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("losses", [False, True])
+def test_the_device_loop_runs_a_job_bit_for_bit_the_host_loop(job_folder, losses):
+    """``--loop device``: the steps on the device in chunks between outputs give the host loop's series, maps and
+    maxima to the bit, the same step count, smallest dt and peaks, the same books to rounding."""
+    from wadi2d_amd.run import run_job
+
+    if losses:
+        _loss_job(job_folder)
+    cfg = jobmod.load_job(job_folder / "job.json")
+    host = run_job(cfg, job_folder / "host", backend="hip-exact")
+    dev = run_job(cfg, job_folder / "dev", backend="hip-exact", loop="device")
+    print(f"device loop (losses {losses}): exit {dev.exit_code} steps {dev.steps}/{host.steps} loop "
+          f"{dev.loop_s:.2f}/{host.loop_s:.2f} s", flush=True)
+    assert host.exit_code == 0 and dev.exit_code == 0, dev.message
+    assert dev.steps == host.steps and dev.t_end == host.t_end and dev.dt_min == host.dt_min
+    _same_outputs(job_folder / "host", job_folder / "dev")
+    assert dev.peaks == host.peaks
+    bh, bd = host.balance, dev.balance
+    # the FP64 books: the device adds each step's own length to the tally, the host loop the difference of its
+    # clock's readings, which can differ in the last bit
+    assert bd.storage == bh.storage
+    assert bd.outflow == pytest.approx(bh.outflow, rel=1e-12) and bd.rain == pytest.approx(bh.rain, rel=1e-12)
+    assert bd.relative == pytest.approx(bh.relative, abs=1e-12)
+
+
+@pytest.mark.gpu
+def test_a_run_stopped_and_resumed_on_the_device_loop_is_the_unbroken_host_run(job_folder):
+    from wadi2d_amd.run import run_job
+
+    cfg = jobmod.load_job(job_folder / "job.json")
+    whole = run_job(cfg, job_folder / "whole", backend="hip-exact")
+    first = run_job(cfg, job_folder / "split", backend="hip-exact", stop_at_s=240.0, loop="device")
+    assert first.exit_code == 6 and first.t_end == pytest.approx(240.0)
+    second = run_job(cfg, job_folder / "split", backend="hip-exact", resume=True, loop="device")
+    print(f"whole {whole.steps} steps, device split {first.steps} + resumed to {second.steps}", flush=True)
+    assert second.exit_code == 0 and second.steps == whole.steps
+    _same_outputs(job_folder / "whole", job_folder / "split")
+
+
 @pytest.mark.gpu
 def test_every_boundary_kind_and_storm_kind_runs_and_a_collapsing_step_ends_in_exit_5(job_folder):
     """A stage and a normal boundary beside the free one, a hyetograph storm, then the same job with a minimum step

@@ -33,7 +33,7 @@ def _run(args: argparse.Namespace) -> int:
     except ValueError as error:
         print(json.dumps({"ev": "error", "code": EXIT_INVALID, "message": str(error)}), file=sys.stderr)
         return EXIT_INVALID
-    return run_job(cfg, args.out, args.backend, resume=args.resume, echo=True).exit_code
+    return run_job(cfg, args.out, args.backend, resume=args.resume, echo=True, loop=args.loop).exit_code
 
 
 def _doctor(_args: argparse.Namespace) -> int:
@@ -60,6 +60,9 @@ def parser() -> argparse.ArgumentParser:
     run.add_argument("--backend", choices=("auto", "gpu", "cpu", "hip", "hip-exact", "hip-f64"), default=None)
     run.add_argument("--dry-run", action="store_true")
     run.add_argument("--resume", action="store_true")
+    run.add_argument("--loop", choices=("host", "device"), default="host",
+                     help="host: the host steps the model, as the reference does; device: the steps run on the device "
+                          "in chunks between outputs, the same results (a HIP backend)")  # fmt: skip
     run.set_defaults(call=_run)
     return top
 

@@ -620,6 +620,8 @@ struct Member {
     double rain_done, rain_window;  // FP64 rain volume: whole past windows, and the current window
     long long nsteps;
     long long scale_off;
+    double dt_min;  // the smallest dt held after a step of this run
+    double t_stop;  // the member holds (steps no further) once its clock has reached this; the host moves it on
     int nwin, win;
     int tile0, tile1;
     int b0, b1, f0, f1;  // the member's boundary cells and tally faces in the handle's lists
@@ -1181,6 +1183,13 @@ __global__ __launch_bounds__(NT) void maxima_kernel(const int* __restrict__ acti
     }
 }
 
+// Cells read by the host (w2d_read_cells): own cell at[q] into out[q].
+template <class T>
+__global__ void gather_kernel(int64_t n, const int64_t* __restrict__ at, const T* __restrict__ src, T* __restrict__ out) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x)
+        out[q] = src[at[q]];
+}
+
 // Cells written from the host (w2d_write_cells): value q into own cell at[q].
 template <class T>
 __global__ void scatter_kernel(int64_t n, const int64_t* __restrict__ at, const T* __restrict__ v, T* __restrict__ dst) {
@@ -1216,6 +1225,7 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.refresh = 0;
     b.rebuild = 0;
     if (b.done) return;
+    if (b.t >= b.t_stop) return;  // held where the host asked (its end too): it reads this state before the next step
     if (b.failed || b.t >= b.t_end - 1e-9) {
         b.done = 1;
         b.rebuild = 3;
@@ -1264,7 +1274,7 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
 // Whether ctl_begin opens a window of this member before its next step (the clock has reached the window's end and
 // the member goes on).
 __device__ __forceinline__ bool window_due(const Member& b) {
-    return !b.done && !b.failed && b.t < b.t_end - 1e-9 && b.t >= b.window_end - 1e-9;
+    return !b.done && !b.failed && b.t < b.t_end - 1e-9 && b.t < b.t_stop && b.t >= b.window_end - 1e-9;
 }
 
 // The SCS cumulative excess of depth p, mm (hydrology/losses.py:59-72): S = (1000 / cn - 10) 25.4, Ia = 0.2 S,
@@ -1791,6 +1801,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
             if (b.dt < num.dt_min) b.failed = 2;
         }
     }
+    b.dt_min = fmin(b.dt_min, b.dt);
     if (before % num.active_every == 0) b.check = 1;
 }
 
@@ -1917,6 +1928,12 @@ struct w2d_handle {
     unsigned long long* run_wave = nullptr;  // per member, the step's largest wave speed as bits
     void** run_faces_ptrs = nullptr;  // [parity][cur 4, other 4]
     int run_maxima = 0;
+    void* cells_buf = nullptr;  // w2d_read_cells: the indices and values of a gather, grown as needed
+    size_t cells_cap = 0;
+    // a lone member's step count and the buffer parities at setup: steps enqueued while it is held (w2d_run_pause)
+    // swap nothing on the device, so its parities follow the steps it took, not the steps enqueued
+    long long run_nsteps0 = 0;
+    int run_hcur0 = 0, run_ucur0 = 0;
     std::vector<double> weight_host;
     // the excess form of the source (w2d_source_losses): src_weight holds depth factors, src_cn curve numbers
     int excess = 0;
@@ -2372,17 +2389,40 @@ int w2d_get_field(w2d_handle* H, int32_t f, void* host, int64_t n) {
 }
 
 int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, void* out) {
-    if (!H || (n && (!index || !out))) return fail(1, "read_cells: bad arguments");
+    if (!H || n < 0 || (n && (!index || !out))) return fail(1, "read_cells: bad arguments");
     void* src = field_ptr(H, f);
     if (!src) return fail(1, "read_cells: the device keeps no copy of this field");
     int s;
     if (momentum_store(f) && (s = materialize(H))) return s;
+    if (n == 0) return 0;
     const size_t es = esize(H);
+    std::vector<int64_t> at(n);
     for (int64_t m = 0; m < n; ++m) {
-        const int64_t c = own_of(H, index[m]);
-        if (c < 0) return fail(1, "read_cells: an index is not an own cell");
-        HIP_TRY(hipMemcpyAsync((char*)out + m * es, (char*)src + c * es, es, hipMemcpyDeviceToHost, H->stream));
+        at[m] = own_of(H, index[m]);
+        if (at[m] < 0) return fail(1, "read_cells: an index is not an own cell");
     }
+    // one gather on the device and one copy back, whatever the number of cells
+    const size_t need = n * (sizeof(int64_t) + es);
+    if (H->cells_cap < need) {
+        if (H->cells_buf) {
+            H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), H->cells_buf), H->owned.end());
+            (void)hipFree(H->cells_buf);
+            H->cells_buf = nullptr;
+            H->cells_cap = 0;
+        }
+        if ((s = dalloc(H, &H->cells_buf, need))) return s;
+        H->cells_cap = need;
+    }
+    int64_t* d_at = (int64_t*)H->cells_buf;
+    void* d_out = (char*)H->cells_buf + n * sizeof(int64_t);
+    HIP_TRY(hipMemcpyAsync(d_at, at.data(), n * sizeof(int64_t), hipMemcpyHostToDevice, H->stream));
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+    if (es == 8)
+        gather_kernel<double><<<blocks, 256, 0, H->stream>>>(n, d_at, (const double*)src, (double*)d_out);
+    else
+        gather_kernel<float><<<blocks, 256, 0, H->stream>>>(n, d_at, (const float*)src, (float*)d_out);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(out, d_out, n * es, hipMemcpyDeviceToHost, H->stream));
     HIP_TRY(hipStreamSynchronize(H->stream));
     return 0;
 }
@@ -3153,6 +3193,9 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         b.tile0 = d.tile0;
         b.tile1 = d.tile1;
         b.refresh_due = d.refresh_due;
+        b.dry = d.dry != 0;
+        b.dt_min = INFINITY;
+        b.t_stop = INFINITY;
     }
     // tiles that keep themselves active: a source, a boundary cell (model.py:117-122, 145-150); each member's
     // boundary cells and tally faces must be one run of the lists, in member order
@@ -3288,6 +3331,9 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         H->graph_steps = 0;
     }
     H->run_m = M;
+    H->run_nsteps0 = mem[0].nsteps;
+    H->run_hcur0 = H->hcur;
+    H->run_ucur0 = H->ucur;
     HIP_TRY(hipStreamSynchronize(H->stream));
     return 0;
 }
@@ -3336,6 +3382,17 @@ int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
     return 0;
 }
 
+int w2d_run_pause(w2d_handle* H, const double* t_stop, int32_t M) {
+    if (!H || !t_stop || M != H->run_m) return fail(1, "run_pause: bad arguments");
+    std::vector<Member> mem(M);
+    HIP_TRY(hipMemcpyAsync(mem.data(), H->run_mem, M * sizeof(Member), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    for (int m = 0; m < M; ++m) mem[m].t_stop = t_stop[m];
+    HIP_TRY(hipMemcpyAsync(H->run_mem, mem.data(), M * sizeof(Member), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
 int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
     if (!H || !out || M != H->run_m) return fail(1, "run_read: bad arguments");
     std::vector<Member> mem(M);
@@ -3354,10 +3411,17 @@ int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
         o.done = b.done;
         o.failed = b.failed;
         o.refresh_due = b.refresh_due;
+        o.dry = b.dry;
+        o.dt_min = b.dt_min;
         // rain credited pro rata in the current window (monitor.py:197-212)
         const double len = b.window_end - b.window_start;
         const double share = len > 0.0 ? (std::min(b.t, b.window_end) - b.window_start) / len : 0.0;
         o.rain = b.rain_done + b.rain_window * share;
+    }
+    if (M == 1 && !mem[0].done) {
+        const long long taken = mem[0].nsteps - H->run_nsteps0;
+        H->hcur = (int)((H->run_hcur0 + taken) & 1);
+        H->ucur = (int)((H->run_ucur0 + taken) & 1);
     }
     // the protocol's view of the list follows the device's
     H->nact = g.nact;

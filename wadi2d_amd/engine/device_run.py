@@ -36,6 +36,7 @@ class MemberPlan:
     nsteps0: int = 0
     refresh_due: bool = True
     storage_index: np.ndarray | None = None
+    dry: bool = False  # the last window's rain was nothing (a resumed run's state)
 
 
 def window_bounds(t0: float, t_end: float, every: float) -> list[tuple[float, float]]:
@@ -72,7 +73,7 @@ class DeviceRun:
         for p in plans:
             sc = window_scales(p.rain, p.t0, p.t_end, source_every_s)
             members.append(Member(p.tile0, p.tile1, p.t0, p.t_end, p.dt0, source_every_s, sc.size, off, p.nsteps0,
-                                  int(p.refresh_due)))  # fmt: skip
+                                  int(p.refresh_due), int(p.dry)))  # fmt: skip
             scales.append(sc)
             off += sc.size
         self.scales = np.concatenate(scales) if scales else np.zeros(0)

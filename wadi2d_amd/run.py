@@ -266,9 +266,14 @@ class Monitor:
 
     def on_step(self, st) -> None:
         self.drv.on_step(st)
-        t = st.t
         if st.dt < self.dt_min:
             self.dt_min = st.dt
+        self.events(st)
+
+    def events(self, st) -> None:
+        """What a step's end may call for: a series sample, a map, a progress event, a checkpoint at the window's
+        end (run.py:1731-1736)."""
+        t = st.t
         if t >= self.next_series:
             self.record(t)
             self.next_series = t + self.cfg.outputs.series_every_s
@@ -345,8 +350,51 @@ class Monitor:
 # the run
 
 
+class _DeviceLoop:
+    """The run's steps on the device (engine/device_run.py) in chunks that hold at the next series sample, map or
+    window end: the device stops stepping the member once its clock has reached that time (w2d_run_pause), so the
+    step that reaches it is the last one taken, and a chunk sized from the current dt rarely wastes a launch. The
+    device keeps the books of every step (tally, maxima every maxima_every steps, rain, the smallest dt); after each
+    chunk the host stepper and the driver take its state, so the monitor, the checkpoint and the summary read what
+    the host loop would have left."""
+
+    GRAPH_STEPS = 16
+    MAX_CHUNK = 64
+
+    def __init__(self, cfg: RunConfig, model, drv: Driver, monitor: Monitor, rain, bset, end: float) -> None:
+        from wadi2d_amd.engine.device_run import DeviceRun, MemberPlan
+
+        if not hasattr(model.backend, "run_setup"):
+            raise InvalidJob("the device loop needs a HIP backend")
+        self.cfg, self.model, self.drv, self.monitor = cfg, model, drv, monitor
+        st = model.stepper
+        plan = MemberPlan(1, model.grid.ntiles, rain, bset, st.t, end, st.dt, st.nsteps, st.refresh_due, rain.state,
+                          dry=drv.dry)  # fmt: skip
+        self.run = DeviceRun(model.backend, [plan], model.num, cfg.run.source_every_s,
+                             maxima=cfg.outputs.maxima_every, idle=st.idle)  # fmt: skip
+        self.rain0 = drv.balance.rain
+
+    def chunk(self, target: float) -> None:
+        st, drv = self.model.stepper, self.drv
+        # the host loop looks after a step: an output due already is taken at the next step's end
+        self.model.backend.run_pause([max(target, float(np.nextafter(st.t, np.inf)))])
+        k = min(self.MAX_CHUNK, max(1, int(np.ceil((target - st.t) / max(st.dt, 1e-9))) + 1))
+        s = self.run.steps(k, self.GRAPH_STEPS)[0]
+        st.t, st.dt, st.nsteps, st.refresh_due = float(s.t), float(s.dt), int(s.nsteps), bool(s.refresh_due)
+        st.idle[:] = self.model.backend.run_idle()
+        drv.balance.rain = self.rain0 + float(s.rain)
+        # the device has credited the rain: the driver's own window adds nothing when it settles
+        drv.window, drv.settled, drv.last_t, drv.dry = (st.t, st.t + 1.0, 0.0), st.t, st.t, bool(s.dry)
+        self.monitor.dt_min = min(self.monitor.dt_min, float(s.dt_min))
+        if s.failed == 1:
+            raise NumericalFailure(f"non-finite wave speed at t = {st.t:.3f} s")
+        if s.failed:
+            raise NumericalFailure(f"time step {st.dt:.2e} s below the minimum at t = {st.t:.3f} s")
+        self.monitor.events(st)
+
+
 def _run(cfg: RunConfig, out: Path, backend: str | None, resume: bool, progress: Progress, summary: RunSummary,
-         stop_files, stop_at_s: float | None = None) -> None:  # fmt: skip
+         stop_files, stop_at_s: float | None = None, loop: str = "host") -> None:  # fmt: skip
     from wadi2d_amd.solver.model import Model
 
     for name in ignored(cfg):
@@ -395,10 +443,16 @@ def _run(cfg: RunConfig, out: Path, backend: str | None, resume: bool, progress:
     try:
         if monitor.stop_requested():
             raise StopRequested
+        device = _DeviceLoop(cfg, model, drv, monitor, rain, bset, end) if loop == "device" else None
         while t < end - 1e-9:
             t_next = min(t + cfg.run.source_every_s, end)
-            drv.open_window(t, t_next)
-            model.advance(t_next, monitor.on_step)
+            if device is None:
+                drv.open_window(t, t_next)
+                model.advance(t_next, monitor.on_step)
+            else:
+                while st.t < t_next - 1e-9:  # the window's end as the host loop's stepper reaches it
+                    maps = monitor.next_map if cfg.outputs.map_every_s > 0.0 else np.inf
+                    device.chunk(min(monitor.next_series, maps, t_next - 1e-9))
             drv.settle(t_next)
             t = t_next
             if monitor.checkpoint_due:
@@ -454,16 +508,18 @@ def _finish(cfg, tables, model, monitor: Monitor, results: Results, out: Path, p
 
 
 def run_job(cfg: RunConfig, out: str | Path, backend: str | None = None, resume: bool = False, echo: bool = False,
-            stop_files: Iterable[Path] = (), stop_at_s: float | None = None) -> RunSummary:  # fmt: skip
+            stop_files: Iterable[Path] = (), stop_at_s: float | None = None, loop: str = "host") -> RunSummary:  # fmt: skip
     """Run one job into folder ``out`` (run.py:1862-1901). ``out/stop`` stops it at a window's end, and so does any
     of ``stop_files``; a stop that was there first stops it at once. ``stop_at_s`` (no counterpart in the reference)
-    stops it as a stop file would, at the first window end at or after that simulated time: a scheduler's slice."""
+    stops it as a stop file would, at the first window end at or after that simulated time: a scheduler's slice.
+    ``loop`` "device" (no counterpart) steps on the device between outputs (``_DeviceLoop``): the same steps and
+    results as the host loop, fewer host round trips."""
     folder = Path(out)
     folder.mkdir(parents=True, exist_ok=True)
     summary = RunSummary()
     with Progress(folder / "progress.jsonl", echo=echo, append=resume) as progress:
         try:
-            _run(cfg, folder, backend, resume, progress, summary, tuple(Path(p) for p in stop_files), stop_at_s)
+            _run(cfg, folder, backend, resume, progress, summary, tuple(Path(p) for p in stop_files), stop_at_s, loop)
             if summary.done:
                 progress.emit("done", **summary.done)
         except JobError as error:

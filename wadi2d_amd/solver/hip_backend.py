@@ -38,7 +38,8 @@ class _Params(ctypes.Structure):
 
 class Member(ctypes.Structure):
     _fields_ = [("tile0", I32), ("tile1", I32), ("t0", D), ("t_end", D), ("dt0", D), ("window", D),
-                ("nwindows", I32), ("scale_offset", I64), ("nsteps0", I64), ("refresh_due", I32)]  # fmt: skip
+                ("nwindows", I32), ("scale_offset", I64), ("nsteps0", I64), ("refresh_due", I32),
+                ("dry", I32)]  # fmt: skip
 
 
 class RunNumerics(ctypes.Structure):
@@ -48,7 +49,7 @@ class RunNumerics(ctypes.Structure):
 
 class MemberState(ctypes.Structure):
     _fields_ = [("t", D), ("dt", D), ("rain", D), ("nsteps", I64), ("done", I32), ("failed", I32),
-                ("refresh_due", I32)]  # fmt: skip
+                ("refresh_due", I32), ("dry", I32), ("dt_min", D)]  # fmt: skip
 
 
 _SIGS = {
@@ -109,6 +110,7 @@ _SIGS = {
     "w2d_run_steps": [VP, I32, I32],
     "w2d_run_read": [VP, ctypes.POINTER(MemberState), I32],
     "w2d_run_idle": [VP, VP],
+    "w2d_run_pause": [VP, VP, I32],
     "w2d_active_tiles": [VP, VP, ctypes.POINTER(I32)],
     "w2d_tally_totals": [VP, VP],
     "w2d_copy_probe": [I64, I32, ctypes.POINTER(D)],
@@ -414,6 +416,11 @@ class HipBackend:
         self.active_tiles = tiles[: n.value].copy()
         self.nact = int(n.value)
         return list(out)
+
+    def run_pause(self, t_stop) -> None:
+        """Hold each member of the run once its clock reaches t_stop[m] (inf: run on)."""
+        ts = _c(np.asarray(t_stop, np.float64).ravel(), np.float64)
+        check(api().w2d_run_pause(self.h, ts.ctypes.data, int(ts.size)))
 
     def run_idle(self) -> np.ndarray:
         out = np.zeros(self.grid.ntiles, np.int32)
