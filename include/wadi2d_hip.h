@@ -71,6 +71,12 @@ int w2d_set_field(w2d_handle* h, int32_t field, const void* host, int64_t n);
 int w2d_get_field(w2d_handle* h, int32_t field, void* host, int64_t n);
 /* Model.read_cells (model.py:139-143): entries at own-cell state indices. */
 int w2d_read_cells(w2d_handle* h, int32_t field, const int64_t* index, int64_t n, void* out);
+/* No counterpart in the reference: w2d_read_cells that allocates nothing, for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc). The index list and the
+ * gathered values go through a file-scope device array of 262,144 entries (4 MiB + 256 B) that each device holds
+ * until the process ends; more cells in one call are refused (status 1). Every call on one device shares that array:
+ * calls that overlap on one device, from several threads or handles, race. Same results as w2d_read_cells. */
+int w2d_read_cells_noalloc(w2d_handle* h, int32_t field, const int64_t* index, int64_t n, void* out);
 /* The writes the reference's callers make through its arrays (momentum.py:49-54 FaceKick.apply into mx; tests into u,
  * v, h): values (the handle's type) into own cells at state indices, under set_field's rules (H both buffers, the
  * momentum stored first and written to both buffers); an index given twice keeps its last value. */

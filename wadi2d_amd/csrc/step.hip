@@ -29,6 +29,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <type_traits>
 #include <vector>
@@ -1190,6 +1191,39 @@ __global__ void gather_kernel(int64_t n, const int64_t* __restrict__ at, const T
         out[q] = src[at[q]];
 }
 
+// The allocation-free gather (w2d_read_cells_noalloc): its index list and values live in `scratch`, a file-scope
+// device array of NOALLOC_CELLS entries that each device holds for the life of the process. Layout: 256 bytes of
+// flags (word 0: an index was not an own cell), then NOALLOC_CELLS int64 state indices, then NOALLOC_CELLS values.
+constexpr int64_t NOALLOC_CELLS = 1 << 18;
+constexpr size_t NOALLOC_FLAGS = 256;
+constexpr size_t NOALLOC_BYTES = NOALLOC_FLAGS + (size_t)NOALLOC_CELLS * (sizeof(int64_t) + sizeof(double));
+__device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+__global__ void noalloc_clear_kernel(int* flags) { flags[0] = 0; }
+
+// State index k (the reference's halo layout) to the own-cell index, or -1 for a halo entry or one out of range;
+// then the value of that own cell into out[q].
+template <class T>
+__global__ void noalloc_gather_kernel(int64_t n, int64_t nstate, const int64_t* __restrict__ k,
+                                      const T* __restrict__ src, T* __restrict__ out, int* __restrict__ flags) {
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < n; q += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = k[q];
+        int64_t own = -1;
+        if (s >= 0 && s < nstate) {
+            const int64_t t = s / STRIDE2, rem = s - t * STRIDE2;
+            const int jj = (int)(rem / STRIDE), ii = (int)(rem - (rem / STRIDE) * STRIDE);
+            if (jj >= HALO && jj < HALO + TILE && ii >= HALO && ii < HALO + TILE)
+                own = t * TILE2 + (jj - HALO) * TILE + (ii - HALO);
+        }
+        if (own < 0) {
+            atomicOr(flags, 1);
+            out[q] = T(0);
+        } else {
+            out[q] = src[own];
+        }
+    }
+}
+
 // Cells written from the host (w2d_write_cells): value q into own cell at[q].
 template <class T>
 __global__ void scatter_kernel(int64_t n, const int64_t* __restrict__ at, const T* __restrict__ v, T* __restrict__ dst) {
@@ -1949,6 +1983,10 @@ struct w2d_handle {
     size_t pix_dep_n = 0;
     std::vector<double> run_pix_host;
     double* run_pix_terms = nullptr;
+    // the device this handle was created on, and its copy of the allocation-free gather's scratch array (the
+    // address hipGetSymbolAddress gave on that device: the array's name is not a device address in host code)
+    int device = 0;
+    void* scratch_dev = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
     int graph_parity = 0;  // the buffer parity the graph was captured at; it replays only at that parity
@@ -1970,6 +2008,26 @@ int dalloc(w2d_handle* H, void** p, size_t bytes) {
     H->owned.push_back(*p);
     e = hipMemsetAsync(*p, 0, bytes ? bytes : 16, H->stream);
     if (e != hipSuccess) return fail(3, std::string("hipMemset: ") + hipGetErrorString(e));
+    return 0;
+}
+
+// Each device's address of `scratch`, looked up once per device, with that device current, at the first handle's
+// set-up on it (w2d_create); every later use takes this address, never the array's name.
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[MAX_DEVICES] = {};
+
+int scratch_address(int dev, void** out) {
+    if (dev < 0 || dev >= MAX_DEVICES) return fail(1, "scratch: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(3, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_addr[dev] = p;
+    }
+    *out = g_scratch_addr[dev];
     return 0;
 }
 
@@ -2257,6 +2315,8 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     };
     if (hipStreamCreateWithFlags(&H->stream, hipStreamNonBlocking) != hipSuccess)
         return bail(fail(3, "hipStreamCreate failed"));
+    if (hipGetDevice(&H->device) != hipSuccess) return bail(fail(3, "hipGetDevice failed"));
+    if ((s = scratch_address(H->device, &H->scratch_dev))) return bail(s);
     const size_t n = H->ncell, es = esize(H);
     for (int b = 0; b < 2; ++b)
         if ((s = dalloc(H, &H->h[b], n * es))) return bail(s);
@@ -2424,6 +2484,41 @@ int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, vo
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipMemcpyAsync(out, d_out, n * es, hipMemcpyDeviceToHost, H->stream));
     HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int w2d_read_cells_noalloc(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, void* out) {
+    if (!H || n < 0 || (n && (!index || !out))) return fail(1, "read_cells_noalloc: bad arguments");
+    if (n > NOALLOC_CELLS)
+        return fail(1, "read_cells_noalloc: more than 262,144 cells in one call (the scratch array's size)");
+    void* src = field_ptr(H, f);
+    if (!src) return fail(1, "read_cells_noalloc: the device keeps no copy of this field");
+    if (!H->scratch_dev) return fail(3, "read_cells_noalloc: no scratch address for this handle's device");
+    int s;
+    if (momentum_store(f) && (s = materialize(H))) return s;
+    if (n == 0) return 0;
+    const size_t es = esize(H);
+    unsigned char* base = (unsigned char*)H->scratch_dev;
+    int* d_flags = (int*)base;
+    int64_t* d_at = (int64_t*)(base + NOALLOC_FLAGS);
+    void* d_val = base + NOALLOC_FLAGS + (size_t)NOALLOC_CELLS * sizeof(int64_t);
+    noalloc_clear_kernel<<<1, 1, 0, H->stream>>>(d_flags);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d_at, index, n * sizeof(int64_t), hipMemcpyHostToDevice, H->stream));
+    const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, 1024);
+    const int64_t nstate = (int64_t)H->ntiles * STRIDE2;
+    if (es == 8)
+        noalloc_gather_kernel<double><<<blocks, 256, 0, H->stream>>>(n, nstate, d_at, (const double*)src,
+                                                                      (double*)d_val, d_flags);
+    else
+        noalloc_gather_kernel<float><<<blocks, 256, 0, H->stream>>>(n, nstate, d_at, (const float*)src,
+                                                                     (float*)d_val, d_flags);
+    HIP_TRY(hipGetLastError());
+    int bad = 0;
+    HIP_TRY(hipMemcpyAsync(out, d_val, n * es, hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipMemcpyAsync(&bad, d_flags, sizeof(int), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    if (bad) return fail(1, "read_cells_noalloc: an index is not an own cell");
     return 0;
 }
 

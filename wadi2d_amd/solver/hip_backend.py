@@ -58,6 +58,7 @@ _SIGS = {
     "w2d_set_field": [VP, I32, VP, I64],
     "w2d_get_field": [VP, I32, VP, I64],
     "w2d_read_cells": [VP, I32, VP, I64, VP],
+    "w2d_read_cells_noalloc": [VP, I32, VP, I64, VP],
     "w2d_write_cells": [VP, I32, VP, I64, VP],
     "w2d_set_active": [VP, VP, I32],
     "w2d_set_boundary": [VP, VP, VP, VP, VP, VP, I32],
@@ -233,6 +234,15 @@ class HipBackend:
         idx = _c(index, np.int64)
         out = np.zeros(idx.size, self.dtype)
         check(api().w2d_read_cells(self.h, FIELDS.index(name), idx.ctypes.data, int(idx.size), out.ctypes.data))
+        return out
+
+    def read_cells_noalloc(self, name: str, index: np.ndarray) -> np.ndarray:
+        """``read_cells`` through the library's fixed device scratch array: no device allocation, at most 262,144
+        cells per call."""
+        idx = _c(index, np.int64)
+        out = np.zeros(idx.size, self.dtype)
+        check(api().w2d_read_cells_noalloc(self.h, FIELDS.index(name), idx.ctypes.data, int(idx.size),
+                                           out.ctypes.data))  # fmt: skip
         return out
 
     def set_boundary(self, bc) -> None:
