@@ -180,6 +180,7 @@ def main() -> int:
     value = domain_cells * steps / wall
     face_s = faces_ms / 1e3 / launches
     cell_s = cells_ms / 1e3 / launches
+    kernel = "faces_kernel"  # the dominant kernel
     face_achieved = FACE_BYTES * tiles * 1024 / face_s if face_s > 0 else 0.0
     log(f"[bench] timed {steps} steps in {wall:.3f} s ({wall / steps * 1e3:.3f} ms/step), simulated "
         f"{tsim:.0f} s; faces {face_s * 1e3:.3f} ms, cells {cell_s * 1e3:.3f} ms per launch; "
@@ -219,12 +220,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "faces_kernel",
+            "kernel": kernel,
             "achieved": face_achieved / 1e9,
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": face_achieved / HBM_PEAK,
-            "traffic": measured_traffic(args, tiles),
+            "traffic": measured_traffic(args, tiles, kernel),
         },
         "cpu_baseline": cpu,
     }
@@ -445,17 +446,17 @@ def learned(args) -> int:
     return 0
 
 
-def measured_traffic(args, tiles: int):
-    """HBM bytes per face-kernel launch from the committed PMC summary of this workload (FETCH_SIZE and WRITE_SIZE,
+def measured_traffic(args, tiles: int, kernel: str):
+    """HBM bytes per launch of ``kernel`` from the committed PMC summary of this workload (FETCH_SIZE and WRITE_SIZE,
     each scaled by what the counter reads for a copy of known size), or None for another workload or launch."""
     path = os.path.join(ROOT, "profiles", "traffic_full_extent_4096.json")
     if args.size != 4096 or not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
-    if tiles != 16641 or rec["launch"]["faces_threads"] != args.faces_threads:
+    if tiles != 16641 or rec["launch"]["faces_threads"] != args.faces_threads or kernel not in rec["per_launch"]:
         return None
-    return rec["per_launch"]["faces_kernel"]["bytes"]
+    return rec["per_launch"][kernel]["bytes"]
 
 
 def cpu_baseline(model, tables, num, seconds: float, dt: float):

@@ -188,15 +188,19 @@ int w2d_run_pixel_terms(w2d_handle* h, const double* depths, int64_t nwindows, i
 /* Set up a run from the handle's current state, active list and the stepper's idle counts (ntiles or null). */
 int w2d_run_setup(w2d_handle* h, const w2d_member* members, int32_t M, const double* scales, int64_t nscales,
                   const w2d_run_numerics* num, const int32_t* idle);
-/* Enqueue nsteps steps of every live member; with graph_steps > 0 (even) they are replayed from one captured graph
- * of that many steps (not while kernel timing is on: HIP does not time events inside graphs). Nothing waits. */
+/* Enqueue nsteps steps of every live member; with graph_steps > 0 they are replayed from one captured graph of that
+ * many steps (not while kernel timing is on: HIP does not time events inside graphs). Nothing waits. Each member keeps
+ * its own buffer parity on the device (the steps it took), so a member that is held or done while others step keeps
+ * its state where it is, and one graph serves every parity. */
 int w2d_run_steps(w2d_handle* h, int32_t nsteps, int32_t graph_steps);
 /* Wait, and read every member's clock, step, count, rain volume and flags (failed: 1 non-finite wave, 2 step below
- * the minimum). */
+ * the minimum). Like every call that reads or writes the state after run_steps (get_field, read_cells, set_field,
+ * write_cells, set_active, step, tally_add, storage, maxima_update, refresh_plan), it first applies the list check the
+ * last step asked for and brings every member to one buffer parity, so the fields read are each member's own. */
 int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
 /* Hold each member once its clock t >= t_stop[m] (no counterpart: the host reads outputs at the step that reaches
  * an output time, as the host loop's step hook does); +inf lets it run on. Steps enqueued while a member is held
- * change nothing of it. */
+ * change nothing of it, whatever the other members do. */
 int w2d_run_pause(w2d_handle* h, const double* t_stop, int32_t M);
 int w2d_run_idle(w2d_handle* h, int32_t* out);
 /* The active list as the device last left it (out may be null to ask for the count). */

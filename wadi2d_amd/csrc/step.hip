@@ -250,10 +250,53 @@ __device__ __forceinline__ R cell_plan(R hb, R Hc, const float2* __restrict__ ta
 // ------------------------------------------------------------------------------------------------------------
 // device arrays of one model
 
+// Every state buffer of one model. Mean depth and the face arrays have two buffers, the current one and the next:
+// the cell kernel reads the ring of a neighbour tile that another workgroup updates in the same launch, and the face
+// kernel writes the face velocity (and momentum) the next step reads while this step's are still read. The face
+// kernel reads level, discharge and wet share before the cell kernel rewrites them, so those are updated in place.
+// `qx1, qy1` hold the face discharges before the positivity limit, between the two kernels.
 template <class R>
-struct State {
-    R *h, *h_next, *d, *u, *v, *mx, *my, *u1, *v1, *mx1, *my1, *qx, *qy, *qx1, *qy1, *res, *src, *om;
+struct Bufs {
+    R *h[2], *u[2], *v[2], *mx[2], *my[2];
+    R *d, *qx, *qy, *om, *qx1, *qy1, *res, *src;
 };
+
+// A value every lane of the wave holds alike, kept in a scalar register (the compiler cannot see that a value loaded
+// from a tile's member is uniform, and would otherwise carry every buffer pointer chosen from it in vector registers).
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// One step's view of the buffers at parity `par` (the steps taken, mod 2): the current and the next buffer of each
+// buffered array (the next mean depth is also the one the last step read, from which the face kernel recomputes the
+// momentum before the cell kernel overwrites it).
+template <class R>
+struct Cur {
+    R *h, *h_next, *u, *v, *u1, *v1, *mx, *my, *d, *qx, *qy, *om, *qx1, *qy1, *res, *src;
+};
+
+template <class R>
+__host__ __device__ __forceinline__ Cur<R> view(const Bufs<R>& b, int par) {
+    // selects between both pointers, not an index into the argument block: the pointers load at the kernel's start
+    // instead of after the parity
+    const bool p = (par & 1) != 0;
+    Cur<R> c;
+    c.h = p ? b.h[1] : b.h[0];
+    c.h_next = p ? b.h[0] : b.h[1];
+    c.u = p ? b.u[1] : b.u[0];
+    c.u1 = p ? b.u[0] : b.u[1];
+    c.v = p ? b.v[1] : b.v[0];
+    c.v1 = p ? b.v[0] : b.v[1];
+    c.mx = p ? b.mx[1] : b.mx[0];
+    c.my = p ? b.my[1] : b.my[0];
+    c.d = b.d;
+    c.qx = b.qx;
+    c.qy = b.qy;
+    c.om = b.om;
+    c.qx1 = b.qx1;
+    c.qy1 = b.qy1;
+    c.res = b.res;
+    c.src = b.src;
+    return c;
+}
 
 struct Statics {
     const float* hc;          // ncell
@@ -628,6 +671,7 @@ struct Member {
     int b0, b1, f0, f1;  // the member's boundary cells and tally faces in the handle's lists
     int stepping, refresh, refresh_due, new_window, dry, rebuild, check, done, failed;
     int nlist;  // the member's active tiles, kept at seg[tile0 .. tile0 + nlist)
+    int par;  // the member's buffer parity (see Cur): a member that does not step keeps its own
 };
 
 struct RunGlobals {
@@ -644,7 +688,8 @@ struct RunNum {
 
 template <class R>
 struct StepArgs {
-    State<R> s;
+    Bufs<R> b;
+    int par;  // a protocol step's buffer parity (a device-resident run takes each member's)
     Statics st;
     Params P;
     const int* active;
@@ -667,7 +712,6 @@ struct StepArgs {
     unsigned long long* member_wave;
     // face momentum: fresh says mx, my hold it (set from the host); else it is the last step's Abar * u, Abar from
     // the mean depths that step read (h_old), recomputed here (a step never stores it)
-    const R* h_old;
     int fresh;
 };
 
@@ -684,23 +728,25 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
     __shared__ R red[NTH / 64];
     const int slot = xcd_slot(blockIdx.x, A.members ? A.glob->nact : A.nact);
     if (slot < 0) return;
-    const int t = A.active[slot];
+    const int t = uni(A.active[slot]);
     R dt_run = A.dt;
     int want_run = A.want_wave;
-    int member = 0;
+    int member = 0, par = A.par;
     if (A.members) {
-        member = A.tile_member[t];
+        member = uni(A.tile_member[t]);
         const Member& mb = A.members[member];
-        if (!mb.stepping) return;
+        if (!uni(mb.stepping)) return;
         dt_run = (R)mb.dt_step;
-        want_run = mb.refresh;
+        want_run = uni(mb.refresh);
+        par = uni(mb.par);
     }
+    const Cur<R> s = view(A.b, par);
     load_nb3(t, A.st.nbr, nb);
     __syncthreads();
     {
         const Stager<NTH> sg = make_stager<NTH>(t, nb);
         R* const dst[6] = {s_u, s_v, s_h, s_d, s_qx, s_qy};
-        const R* const src[6] = {A.s.u, A.s.v, A.s.h, A.s.d, A.s.qx, A.s.qy};
+        const R* const src[6] = {s.u, s.v, s.h, s.d, s.qx, s.qy};
         stage<R, 6, NTH>(dst, src, sg);
     }
     __syncthreads();
@@ -731,23 +777,23 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
         // same operations on the same operands as its own product, so the same bits as a stored copy
         R mxo, myo;
         if (fresh) {
-            mxo = A.s.mx[c];
-            myo = A.s.my[c];
+            mxo = s.mx[c];
+            myo = s.my[c];
         } else {
-            const R ho = ld_nt(A.h_old + c);
-            mxo = R(0.5) * (A.h_old[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
-            myo = R(0.5) * (A.h_old[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
+            const R ho = ld_nt(s.h_next + c);
+            mxo = R(0.5) * (s.h_next[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
+            myo = R(0.5) * (s.h_next[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
         }
         const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), efx, cwx, A.st.zcx, A.st.ftabx + c,
                                             A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
-                                            A.s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
-        st_nt(un, A.s.u1 + c);
-        st_nt(qn, A.s.qx1 + c);
+                                            s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
+        st_nt(un, s.u1 + c);
+        st_nt(qn, s.qx1 + c);
         const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), efy, cwy, A.st.zcy, A.st.ftaby + c,
                                             A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
-                                            A.s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
-        st_nt(un, A.s.v1 + c);
-        st_nt(qn, A.s.qy1 + c);
+                                            s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
+        st_nt(un, s.v1 + c);
+        st_nt(qn, s.qy1 + c);
         const R m = wx > wy ? wx : wy;
         w = m > w ? m : w;
     }
@@ -787,8 +833,7 @@ struct ClosureArgs {
 constexpr int CLOSURE_STATIC = 11;
 constexpr float CLOSURE_LOGC = 1.5f, CLOSURE_RISE = 4.f;
 
-__global__ __launch_bounds__(256) void closure_kernel(ClosureArgs C, const float* __restrict__ d,
-                                                      const float* __restrict__ h) {
+__global__ __launch_bounds__(256) void closure_kernel(ClosureArgs C, Bufs<float> b, int par) {
     using namespace w2d_mlp;
     __shared__ Net net;
     __shared__ __align__(16) uint16_t xs[4][WAVE_SAMPLES * IN];
@@ -812,6 +857,10 @@ __global__ __launch_bounds__(256) void closure_kernel(ClosureArgs C, const float
             float x[IN];
             if (f < C.n) {
                 const int r = C.cell_r[f], l = C.cell_l[f];
+                // the current level and depth of the face's member (a protocol step: the handle's parity)
+                const int pr = C.members ? C.members[C.tile_member[r / TILE2]].par : par;
+                const float* __restrict__ d = b.d;
+                const float* __restrict__ h = b.h[pr & 1];
                 const float4 st = C.axis[f] ? C.sx[r] : C.sy[r];
                 const float* s = C.stat + (size_t)f * CLOSURE_STATIC;
                 for (int q = 0; q < CLOSURE_STATIC; ++q) x[q] = s[q];
@@ -843,59 +892,40 @@ __global__ __launch_bounds__(256) void closure_kernel(ClosureArgs C, const float
 template <class R>
 __device__ __forceinline__ R scaled(R q, R rl, R rr) { return q * (q > R(0) ? rl : rr); }
 
-// Positivity ratio on the tile and a one-cell ring (cells.cu:4-18), then continuity, rain, the compensated
-// residual, the level and wet share of every own cell (cells.cu:28-63).
-template <class R, bool FAST, int NTH>
-__global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
-    __shared__ __align__(16) R s_h[LSIZE];
-    __shared__ __align__(16) R s_qx[LSIZE];
-    __shared__ __align__(16) R s_qy[LSIZE];
-    constexpr int RW = TILE + 2;  // the ring of r: cells -1 .. 32, compact (7 workgroups per CU fit the LDS)
-    __shared__ __align__(16) R s_r[RW * RW];
-    __shared__ int nb[9];
-    __shared__ int any_wet;
-    __shared__ double red_src[NTH / 64];
-    const int slot = xcd_slot(blockIdx.x, A.members ? A.glob->nact : A.nact);
-    if (slot < 0) return;
-    const int t = A.active[slot];
-    R dt_run = A.dt;
-    int want_run = A.want_wave;
-    int new_window = 0;
-    double scale = 0.0;
-    if (A.members) {
-        const Member& mb = A.members[A.tile_member[t]];
-        if (!mb.stepping) return;
-        dt_run = (R)mb.dt_step;
-        want_run = mb.refresh;
-        new_window = mb.new_window && !A.excess;
-        scale = mb.scale;
-    }
-    load_nb3(t, A.st.nbr, nb);
-    if (threadIdx.x == 0) any_wet = 0;
-    __syncthreads();
-    {
-        const Stager<NTH> sg = make_stager<NTH>(t, nb);
-        R* const dst[3] = {s_h, s_qx, s_qy};
-        const R* const src[3] = {A.s.h, A.s.qx1, A.s.qy1};
-        stage<R, 3, NTH>(dst, src, sg);
-    }
-    __syncthreads();
-    const R dx = R(A.P.dx);
-    const R area = dx * dx;
-    const R dt = dt_run;
+// LDS index of tile cell (j, i) in a block whose row 0 holds tile row -ROW0 (LS entries per row, own cells from
+// column 4): ROW0 = 2 for the two-pass kernels' blocks (lds_of), 3 for the fused step's.
+template <int ROW0>
+__device__ __forceinline__ int lidx(int j, int i) { return (j + ROW0) * LS + i + 4; }
+
+// The positivity ratio (cells.cu:4-18) of the tile and a one-cell ring, into s_r (34 x 34, cell (j, i) at
+// (j + 1) * 34 + i + 1), from the mean depth and the face discharges before the limit in LDS. The four corner cells
+// are never read; CORNERS says whether their faces are in LDS (the fused step computes no face beyond the ring).
+constexpr int RW = TILE + 2;
+template <class R, bool FAST, int NTH, int ROW0, bool CORNERS>
+__device__ __forceinline__ void limit_ring(const R* s_h, const R* s_qx, const R* s_qy, R* s_r, R dt, R area) {
     for (int p = threadIdx.x; p < RW * RW; p += NTH) {
         const int jj = p / RW, ii = p - (p / RW) * RW;
-        const int L = lds_of(jj - 1, ii - 1);
+        if (!CORNERS && (jj == 0 || jj == RW - 1) && (ii == 0 || ii == RW - 1)) continue;
+        const int L = lidx<ROW0>(jj - 1, ii - 1);
         const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + LS], R(0)) +
                       pmax(-s_qy[L], R(0));
         const R have = pmax(s_h[L], R(0)) * area;
         s_r[p] = out * dt > have ? dv<FAST>(have, out * dt) : R(1);
     }
-    __syncthreads();
+}
+
+// Continuity, rain, the compensated residual, the level and the wet share of the tile's own cells
+// (cells.cu:28-63), from the ratio and the face discharges before the limit in LDS: writes the limited discharges,
+// the new mean depth, level and residual, the wet share on a wave step, the window's source; returns whether a cell
+// is wet and adds the window's source to src_sum.
+template <class R, bool FAST, int NTH, int ROW0>
+__device__ __forceinline__ bool update_tile(const StepArgs<R>& A, const Cur<R>& s, int t, const R* s_h,
+                                            const R* s_qx, const R* s_qy, const R* s_r, R dt, bool want_plan,
+                                            bool new_window, double scale, double& src_sum) {
+    const R dx = R(A.P.dx);
+    const R area = dx * dx;
     const int i = threadIdx.x & (TILE - 1);
-    const bool want_plan = want_run != 0;
-    int wet = 0;
-    double src_sum = 0.0;
+    bool wet = false;
     // The thread's RT cells in three passes, so that the loads of one cell need not wait for the stores of the
     // one before: (1) the cell's inputs, (2) continuity and the level table's row pair, (3) the stores.
     constexpr int RT = TILE2 / NTH;
@@ -908,13 +938,13 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
         cs[r] = c;
-        res_v[r] = ld_nt(A.s.res + c);
+        res_v[r] = ld_nt(s.res + c);
         hc_v[r] = ld_nt(A.st.hc + c);
         tag_v[r] = ld_nt(A.st.ctag + c);
         if (new_window)  // the window's rate of this cell: float(weight * scale), the product in double
             src_v[r] = A.src_weight ? (R)(A.src_weight[c] * scale) : R(0);
         else
-            src_v[r] = ld_nt(A.s.src + c);
+            src_v[r] = ld_nt(s.src + c);
     }
     R qw_v[RT], qn_v[RT], hn_v[RT], resn_v[RT], s_v[RT];
     int row_v[RT], mode_v[RT];
@@ -922,7 +952,7 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
-        const int L = lds_of(j, i);
+        const int L = lidx<ROW0>(j, i);
         const int Q = (j + 1) * RW + (i + 1);  // the cell in s_r
         const unsigned c = cs[r];
         const R rc = s_r[Q];
@@ -939,8 +969,8 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
         resn_v[r] = sum > R(0) ? inc - (sum - h0) : R(0);
         hn_v[r] = hn;
         const R Hc = R(hc_v[r]);
-        // the level table's row for hn (the statements of cell_level and cell_plan below, the row pair read
-        // through the cell's row cache): 0 no table, 1 above its top, 2 within it
+        // the level table's row for hn (the statements of cell_level and cell_plan, the row pair read through the
+        // cell's row cache): 0 no table, 1 above its top, 2 within it
         int mode = 0, row = 0;
         R sv = R(0);
         if (Hc > R(0)) {
@@ -961,15 +991,15 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const unsigned c = cs[r];
-        st_nt(qw_v[r], A.s.qx + c);
-        st_nt(qn_v[r], A.s.qy + c);
+        st_nt(qw_v[r], s.qx + c);
+        st_nt(qn_v[r], s.qy + c);
         if (new_window) {
-            A.s.src[c] = src_v[r];
+            s.src[c] = src_v[r];
             src_sum += (double)src_v[r];
         }
-        st_nt(resn_v[r], A.s.res + c);
+        st_nt(resn_v[r], s.res + c);
         const R hn = hn_v[r];
-        st_nt(hn, A.s.h_next + c);
+        st_nt(hn, s.h_next + c);
         const R Hc = R(hc_v[r]);
         const float2 q = pair_v[r];
         R dn, plan = R(1);
@@ -989,24 +1019,99 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
             A.st.cc[c] = q;
             A.st.ctag[c] = (unsigned char)row_v[r];
         }
-        st_nt(dn, A.s.d + c);
-        if (want_plan) st_nt(plan, A.s.om + c);
+        st_nt(dn, s.d + c);
+        if (want_plan) st_nt(plan, s.om + c);
         wet |= hn > R(0);
     }
-    if (wet) any_wet = 1;
+    return wet;
+}
+
+// The tile's wet flag and, on a new window, its source sum (one per tile), after update_tile.
+template <int NTH>
+__device__ __forceinline__ void tile_flags(int t, bool wet, bool new_window, double src_sum, int* any_wet,
+                                           double* red_src, int* wet_out, double* src_partial) {
+    if (wet) *any_wet = 1;
     if (new_window) {
         for (int off = 32; off > 0; off >>= 1) src_sum += __shfl_xor(src_sum, off);
         if ((threadIdx.x & 63) == 0) red_src[threadIdx.x >> 6] = src_sum;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        A.wet[t] = any_wet;
+        wet_out[t] = *any_wet;
         if (new_window) {
             double sum = red_src[0];
             for (int w = 1; w < NTH / 64; ++w) sum += red_src[w];
-            A.src_partial[t] = sum;
+            src_partial[t] = sum;
         }
     }
+}
+
+// The run's per-tile step parameters: this step's length, whether it refreshes the wave speed, whether it opens a
+// source window (and its scale), the buffer parities; false for a tile whose member does not step.
+template <class R>
+struct TileStep {
+    R dt;
+    int want, new_window, member, par;
+    long long nsteps;
+    double scale;
+};
+template <class R>
+__device__ __forceinline__ bool tile_step(const StepArgs<R>& A, int t, TileStep<R>& ts) {
+    ts.dt = A.dt;
+    ts.want = A.want_wave;
+    ts.new_window = 0;
+    ts.scale = 0.0;
+    ts.member = 0;
+    ts.par = A.par;
+    ts.nsteps = 0;
+    if (A.members) {
+        ts.member = uni(A.tile_member[t]);
+        const Member& mb = A.members[ts.member];
+        if (!uni(mb.stepping)) return false;
+        ts.dt = (R)mb.dt_step;
+        ts.want = uni(mb.refresh);
+        ts.new_window = uni(mb.new_window) && !A.excess;
+        ts.scale = mb.scale;
+        ts.par = uni(mb.par);
+        ts.nsteps = mb.nsteps;
+    }
+    return true;
+}
+
+// Positivity ratio on the tile and a one-cell ring (cells.cu:4-18), then continuity, rain, the compensated
+// residual, the level and wet share of every own cell (cells.cu:28-63): the second kernel of the two-pass step.
+template <class R, bool FAST, int NTH>
+__global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
+    __shared__ __align__(16) R s_h[LSIZE];
+    __shared__ __align__(16) R s_qx[LSIZE];
+    __shared__ __align__(16) R s_qy[LSIZE];
+    __shared__ __align__(16) R s_r[RW * RW];  // compact (7 workgroups per CU fit the LDS)
+    __shared__ int nb[9];
+    __shared__ int any_wet;
+    __shared__ double red_src[NTH / 64];
+    const int slot = xcd_slot(blockIdx.x, A.members ? A.glob->nact : A.nact);
+    if (slot < 0) return;
+    const int t = uni(A.active[slot]);
+    TileStep<R> ts;
+    if (!tile_step(A, t, ts)) return;
+    const Cur<R> s = view(A.b, ts.par);
+    load_nb3(t, A.st.nbr, nb);
+    if (threadIdx.x == 0) any_wet = 0;
+    __syncthreads();
+    {
+        const Stager<NTH> sg = make_stager<NTH>(t, nb);
+        R* const dst[3] = {s_h, s_qx, s_qy};
+        const R* const src[3] = {s.h, s.qx1, s.qy1};
+        stage<R, 3, NTH>(dst, src, sg);
+    }
+    __syncthreads();
+    const R dx = R(A.P.dx);
+    limit_ring<R, FAST, NTH, 2, true>(s_h, s_qx, s_qy, s_r, ts.dt, dx * dx);
+    __syncthreads();
+    double src_sum = 0.0;
+    const bool wet = update_tile<R, FAST, NTH, 2>(A, s, t, s_h, s_qx, s_qy, s_r, ts.dt, ts.want != 0,
+                                                  ts.new_window != 0, ts.scale, src_sum);
+    tile_flags<NTH>(t, wet, ts.new_window != 0, src_sum, &any_wet, red_src, A.wet, A.src_partial);
 }
 
 // Wet share of the listed tiles' cells (cells.cu:66-77).
@@ -1042,12 +1147,36 @@ __global__ void boundary_kernel(int n, const int* __restrict__ cell, const int* 
     }
 }
 
-// Copy whole tiles between the two mean-depth buffers: a tile that leaves the active list keeps one value.
+// Tiles that leave the active list (a protocol set_active) keep one mean depth in both buffers, at the handle's parity.
 template <class R>
-__global__ void copy_tiles_kernel(const int* __restrict__ tiles, int n, const R* __restrict__ a, R* __restrict__ b) {
+__global__ void leave_tiles_kernel(const int* __restrict__ tiles, int n, Bufs<R> B, int par) {
     if ((int)blockIdx.x >= n) return;
+    const Cur<R> s = view(B, par);
     const size_t base = (size_t)tiles[blockIdx.x] * TILE2;
-    for (int p = threadIdx.x; p < TILE2; p += NT) b[base + p] = a[base + p];
+    for (int p = threadIdx.x; p < TILE2; p += NT) s.h_next[base + p] = s.h[base + p];
+}
+
+template <class R>
+__device__ __forceinline__ void swap_at(R* a, R* b, size_t c) {
+    const R x = a[c], y = b[c];
+    a[c] = y;
+    b[c] = x;
+}
+
+// Moves the buffered arrays of tiles [t0, t1) from parity `par` to `par2`: each buffer's content goes where the same
+// role sits at the new parity (a swap, so a sleeping tile's second velocity buffer keeps its meaning), so that a member
+// that took a different number of steps than the handle reads as the handle does.
+template <class R>
+__global__ void align_kernel(int t0, int t1, Bufs<R> B, int par, int par2) {
+    if ((par & 1) == (par2 & 1)) return;
+    const size_t c0 = (size_t)t0 * TILE2, c1 = (size_t)t1 * TILE2;
+    for (size_t c = c0 + blockIdx.x * (size_t)blockDim.x + threadIdx.x; c < c1; c += (size_t)gridDim.x * blockDim.x) {
+        swap_at(B.h[0], B.h[1], c);
+        swap_at(B.u[0], B.u[1], c);
+        swap_at(B.v[0], B.v[1], c);
+        swap_at(B.mx[0], B.mx[1], c);
+        swap_at(B.my[0], B.my[1], c);
+    }
 }
 
 // The face momenta mx, my of the listed tiles (all active tiles: n, or glob->nact in a device-resident run) from the
@@ -1559,13 +1688,13 @@ __device__ void rebuild(Member& b, int mode, const RunLists& L, int (*wave_cnt)[
 }
 
 // What a change of the list does to tiles (stepper.py:94-99): a woken tile's wet share from its depth, by the
-// kernel's own arithmetic; a tile that leaves keeps one mean depth in both buffers, and a finished member's tile
-// also its face buffers.
+// kernel's own arithmetic; a tile that leaves keeps one mean depth in both buffers, and a finished member's tile also
+// one velocity.
 template <class R, bool FAST>
-__device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, R* __restrict__ h,
-                            R* __restrict__ h_other, R* __restrict__ om, R* const* __restrict__ faces_cur,
-                            R* const* __restrict__ faces_other, const float* __restrict__ hc,
-                            const float2* __restrict__ ctab, size_t ncell, int last, R dx, int fresh) {
+__device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, const Bufs<R>& B,
+                            const float* __restrict__ hc, const float2* __restrict__ ctab, size_t ncell, int last,
+                            R dx, int fresh) {
+    const Cur<R> s = view(B, b.par);
     // tiles are taken TB at a time, every load of a batch issued before its stores (a member that finishes moves all
     // its tiles here at once)
     constexpr int TB = 4;
@@ -1576,16 +1705,17 @@ __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, 
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
             cv[u] = q0 + u < nw ? (size_t)L.woken[b.tile0 + q0 + u] * TILE2 + p : 0;
-            hv[u] = q0 + u < nw ? h[cv[u]] : R(0);
+            hv[u] = q0 + u < nw ? s.h[cv[u]] : R(0);
             hcv[u] = q0 + u < nw ? R(hc[cv[u]]) : R(0);
         }
 #pragma unroll
         for (int u = 0; u < TB; ++u)
-            if (q0 + u < nw) om[cv[u]] = cell_plan<R, FAST>(hv[u], hcv[u], ctab + cv[u], ncell, last);
+            if (q0 + u < nw) s.om[cv[u]] = cell_plan<R, FAST>(hv[u], hcv[u], ctab + cv[u], ncell, last);
     }
     // a leaving tile's face momentum is stored (both buffers) from the depths the last step read, before those
     // depth buffers are made one; a stored momentum (set from the host since) is already in both
     if (nl && !fresh) {
+        const int p2 = b.par & 1;
         for (int q0 = 0; q0 < nl; q0 += TB) {
             R av[TB], ev[TB];
             size_t cv[TB];
@@ -1597,14 +1727,14 @@ __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, 
                 const size_t w = i > 0 ? c - 1 : (size_t)L.nbr[(size_t)t * 8 + 3] * TILE2 + j * TILE + (TILE - 1);
                 const size_t n = j > 0 ? c - TILE : (size_t)L.nbr[(size_t)t * 8 + 1] * TILE2 + (TILE - 1) * TILE + i;
                 cv[u] = c;
-                av[u] = R(0.5) * (h_other[w] + h_other[c]) * dx * faces_cur[0][c];
-                ev[u] = R(0.5) * (h_other[n] + h_other[c]) * dx * faces_cur[1][c];
+                av[u] = R(0.5) * (s.h_next[w] + s.h_next[c]) * dx * s.u[c];
+                ev[u] = R(0.5) * (s.h_next[n] + s.h_next[c]) * dx * s.v[c];
             }
 #pragma unroll
             for (int u = 0; u < TB; ++u) {
                 if (q0 + u >= nl) continue;
-                faces_cur[2][cv[u]] = av[u], faces_other[2][cv[u]] = av[u];
-                faces_cur[3][cv[u]] = ev[u], faces_other[3][cv[u]] = ev[u];
+                B.mx[p2][cv[u]] = av[u], B.mx[1 - p2][cv[u]] = av[u];
+                B.my[p2][cv[u]] = ev[u], B.my[1 - p2][cv[u]] = ev[u];
             }
         }
         __syncthreads();
@@ -1617,14 +1747,14 @@ __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, 
         for (int u = 0; u < TB; ++u) {
             if (q0 + u >= nl) continue;
             cv[u] = (size_t)L.leaving[b.tile0 + q0 + u] * TILE2 + p;
-            hv[u] = h[cv[u]];
-            if (b.done) uv[u] = faces_cur[0][cv[u]], vv[u] = faces_cur[1][cv[u]];
+            hv[u] = s.h[cv[u]];
+            if (b.done) uv[u] = s.u[cv[u]], vv[u] = s.v[cv[u]];
         }
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
             if (q0 + u >= nl) continue;
-            h_other[cv[u]] = hv[u];
-            if (b.done) faces_other[0][cv[u]] = uv[u], faces_other[1][cv[u]] = vv[u];
+            s.h_next[cv[u]] = hv[u];
+            if (b.done) s.u1[cv[u]] = uv[u], s.v1[cv[u]] = vv[u];
         }
     }
     __syncthreads();
@@ -1699,10 +1829,7 @@ __device__ void gather_lists(const Member* __restrict__ mem, int M, RunGlobals* 
 template <class R, bool FAST>
 __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ mem, RunGlobals* __restrict__ g,
                                                           const double* __restrict__ scales, const RunNum num,
-                                                          RunLists L, R* __restrict__ h, R* __restrict__ h_other,
-                                                          R* __restrict__ om, R* const* __restrict__ faces_cur,
-                                                          R* const* __restrict__ faces_other,
-                                                          const float* __restrict__ hc,
+                                                          RunLists L, Bufs<R> B, const float* __restrict__ hc,
                                                           const float2* __restrict__ ctab, size_t ncell, int last,
                                                           R dx, int fresh, unsigned long long* __restrict__ member_wave,
                                                           const double* __restrict__ src_partial, int flush_only) {
@@ -1715,7 +1842,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
     bool changed = false;
     if (b.check) {
         rebuild(b, 2, L, wave_cnt, nw, nl, sflags);
-        transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last, dx, fresh);
+        transitions<R, FAST>(b, nw, nl, L, B, hc, ctab, ncell, last, dx, fresh);
         if (threadIdx.x == 0) b.check = 0;
         changed = true;
     }
@@ -1743,7 +1870,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
         __syncthreads();
         if (any) {
             rebuild(b, b.rebuild, L, wave_cnt, nw, nl, sflags);
-            transitions<R, FAST>(b, nw, nl, L, h, h_other, om, faces_cur, faces_other, hc, ctab, ncell, last, dx, fresh);
+            transitions<R, FAST>(b, nw, nl, L, B, hc, ctab, ncell, last, dx, fresh);
             changed = true;
         }
     }
@@ -1780,14 +1907,17 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
                                                    const double* __restrict__ src_partial, const RunNum num,
                                                    const int* __restrict__ cell, const int* __restrict__ ref,
                                                    const int* __restrict__ kind, const float* __restrict__ d0,
-                                                   const float* __restrict__ h0, R* __restrict__ h, R* __restrict__ d,
+                                                   const float* __restrict__ h0, Bufs<R> B,
                                                    const int* __restrict__ face, const int* __restrict__ axis,
-                                                   const float* __restrict__ sign, const R* __restrict__ qx,
-                                                   const R* __restrict__ qy, double* __restrict__ total,
+                                                   const float* __restrict__ sign, double* __restrict__ total,
                                                    double* __restrict__ flow) {
     __shared__ double red_s[POST_THREADS / 64];
     Member& b = mem[blockIdx.x];
     if (!b.stepping) return;
+    // the step's outputs: the next buffers of the parities it stepped at (thread 0 advances them below)
+    const Cur<R> s = view(B, b.par);
+    R* const h = s.h_next;
+    R* const d = s.d;
     for (int p = b.b0 + threadIdx.x; p < b.b1; p += blockDim.x) {
         const int k = cell[p];
         if (kind[p] == 0) {
@@ -1802,7 +1932,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
         }
     }
     for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) {
-        const R qq = axis[q] ? qy[face[q]] : qx[face[q]];
+        const R qq = axis[q] ? s.qy[face[q]] : s.qx[face[q]];
         const double f = (double)(R(sign[q]) * qq);
         total[q] += b.dt_step * f;
         flow[q] = f;
@@ -1817,6 +1947,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
         if ((threadIdx.x & 63) == 0) red_s[threadIdx.x >> 6] = rs;
         __syncthreads();
     }
+    __syncthreads();  // every thread has read the parities before thread 0 moves them on
     if (threadIdx.x != 0) return;
     if (b.refresh) w = wave_value<R>(member_wave[blockIdx.x]);
     if (books_rain) {
@@ -1837,6 +1968,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     }
     b.dt_min = fmin(b.dt_min, b.dt);
     if (before % num.active_every == 0) b.check = 1;
+    b.par = 1 - b.par;
 }
 
 // Running maxima after the step, at the member's new time (engine/maxima.py:29-51).
@@ -1844,16 +1976,20 @@ template <class R>
 __global__ __launch_bounds__(NT) void maxima_run_kernel(const RunGlobals* __restrict__ g, const Member* __restrict__ mem,
                                                         const int* __restrict__ tile_member,
                                                         const int* __restrict__ active, const int* __restrict__ nbr,
-                                                        R arrive, R deep, const R* __restrict__ d,
-                                                        const R* __restrict__ h, const R* __restrict__ u,
-                                                        const R* __restrict__ v, R* __restrict__ dmax,
+                                                        R arrive, R deep, Bufs<R> B, R* __restrict__ dmax,
                                                         R* __restrict__ hmax, R* __restrict__ smax,
                                                         R* __restrict__ zmax, R* __restrict__ tarr, int every) {
     if ((int)blockIdx.x >= g->nact) return;
-    const int t = active[blockIdx.x];
-    const Member& b = mem[tile_member[t]];
-    // sampled after the steps whose count is a multiple of `every`, as the host loop's on_step does
+    const int t = uni(active[blockIdx.x]);
+    const Member& b = mem[uni(tile_member[t])];
+    // sampled after the steps whose count is a multiple of `every`, as the host loop's on_step does; the member's
+    // parities have moved on, so its current buffers hold the step's results
     if (!b.stepping || b.nsteps % every != 0) return;
+    const Cur<R> s = view(B, uni(b.par));
+    const R* __restrict__ d = s.d;
+    const R* __restrict__ h = s.h;
+    const R* __restrict__ u = s.u;
+    const R* __restrict__ v = s.v;
     const R time = (R)b.t;
     const int te = nbr[(size_t)t * 8 + 4], ts = nbr[(size_t)t * 8 + 6];
     for (int p = threadIdx.x; p < TILE2; p += NT) {
@@ -1909,16 +2045,15 @@ struct w2d_handle {
     size_t ncell = 0;
     Params P{};
     hipStream_t stream = nullptr;
-    // device state, element size prec
-    void* h[2] = {nullptr, nullptr};
-    int hcur = 0;
-    void* uv[4][2] = {};  // u, v, mx, my: [field][buffer]
-    int ucur = 0;
+    // device state, element size prec: the buffers of Bufs, and the handle's parity (see Cur); in a device-resident
+    // run each member keeps its own, brought back to this one by run_align
+    void *hb[2] = {}, *ub[2] = {}, *vb[2] = {}, *mxb[2] = {}, *myb[2] = {};
+    void *d = nullptr, *qx = nullptr, *qy = nullptr, *om = nullptr;
+    int par = 0;
     // the face momentum: steps do not store it (the face kernel recomputes it from u and the depths the last step
     // read); m_fresh says whether mx, my hold it, as after set_field or materialize; it goes to the kernels by value
     int m_fresh = 1;
-    void *d = nullptr, *qx = nullptr, *qy = nullptr, *qx1 = nullptr, *qy1 = nullptr, *res = nullptr,
-         *src = nullptr, *om = nullptr;
+    void *qx1 = nullptr, *qy1 = nullptr, *res = nullptr, *src = nullptr;
     void* wave = nullptr;
     int* wet = nullptr;
     // statics
@@ -1960,14 +2095,12 @@ struct w2d_handle {
         *run_idle = nullptr, *run_flags = nullptr, *run_woken = nullptr, *run_leaving = nullptr, *run_seg = nullptr;
     double *run_scales = nullptr, *run_src_partial = nullptr;
     unsigned long long* run_wave = nullptr;  // per member, the step's largest wave speed as bits
-    void** run_faces_ptrs = nullptr;  // [parity][cur 4, other 4]
     int run_maxima = 0;
     void* cells_buf = nullptr;  // w2d_read_cells: the indices and values of a gather, grown as needed
     size_t cells_cap = 0;
-    // a lone member's step count and the buffer parities at setup: steps enqueued while it is held (w2d_run_pause)
-    // swap nothing on the device, so its parities follow the steps it took, not the steps enqueued
-    long long run_nsteps0 = 0;
-    int run_hcur0 = 0, run_ucur0 = 0;
+    // steps were enqueued since the members' parities were last brought back to the handle's (run_align), or the
+    // handle stepped on its own since the members took its parities
+    int run_dirty = 0, members_stale = 0;
     std::vector<double> weight_host;
     // the excess form of the source (w2d_source_losses): src_weight holds depth factors, src_cn curve numbers
     int excess = 0;
@@ -1989,7 +2122,6 @@ struct w2d_handle {
     void* scratch_dev = nullptr;
     hipGraphExec_t graph = nullptr;
     int graph_steps = 0;
-    int graph_parity = 0;  // the buffer parity the graph was captured at; it replays only at that parity
 
     // kernel timing: when on, every step records events around its two kernels on the handle's stream
     int timing = 0;
@@ -2040,51 +2172,77 @@ int upload(w2d_handle* H, T** dst, const T* src, size_t count) {
     return 0;
 }
 
-// Device pointer of field f (the current buffer for the swapped ones), or null for a field the device keeps no copy of.
+template <class R>
+Bufs<R> bufs(const w2d_handle* H) {
+    Bufs<R> b;
+    for (int k = 0; k < 2; ++k) {
+        b.h[k] = (R*)H->hb[k];
+        b.u[k] = (R*)H->ub[k];
+        b.v[k] = (R*)H->vb[k];
+        b.mx[k] = (R*)H->mxb[k];
+        b.my[k] = (R*)H->myb[k];
+    }
+    b.d = (R*)H->d;
+    b.qx = (R*)H->qx;
+    b.qy = (R*)H->qy;
+    b.om = (R*)H->om;
+    b.qx1 = (R*)H->qx1;
+    b.qy1 = (R*)H->qy1;
+    b.res = (R*)H->res;
+    b.src = (R*)H->src;
+    return b;
+}
+
+// The handle's current view of its buffers (the element type does not matter for the pointers).
+Cur<char> cur(const w2d_handle* H) { return view(bufs<char>(H), H->par); }
+
+// Device pointer of field f (the current buffer for the buffered ones), or null for a field the device keeps no copy of.
 void* field_ptr(w2d_handle* H, int f) {
+    const Cur<char> c = cur(H);
     switch (f) {
-        case W2D_H: return H->h[H->hcur];
-        case W2D_D: return H->d;
-        case W2D_U: return H->uv[0][H->ucur];
-        case W2D_V: return H->uv[1][H->ucur];
-        case W2D_MX: return H->uv[2][H->ucur];
-        case W2D_MY: return H->uv[3][H->ucur];
-        case W2D_QX: return H->qx;
-        case W2D_QY: return H->qy;
-        case W2D_U1: return H->uv[0][1 - H->ucur];
-        case W2D_V1: return H->uv[1][1 - H->ucur];
-        case W2D_MX1: return H->uv[2][1 - H->ucur];
-        case W2D_MY1: return H->uv[3][1 - H->ucur];
+        case W2D_H: return c.h;
+        case W2D_D: return c.d;
+        case W2D_U: return c.u;
+        case W2D_V: return c.v;
+        case W2D_MX: return c.mx;
+        case W2D_MY: return c.my;
+        case W2D_QX: return c.qx;
+        case W2D_QY: return c.qy;
+        case W2D_U1: return c.u1;
+        case W2D_V1: return c.v1;
+        case W2D_MX1: return H->mxb[1 - (H->par & 1)];
+        case W2D_MY1: return H->myb[1 - (H->par & 1)];
         case W2D_QX1: return H->qx1;
         case W2D_QY1: return H->qy1;
         case W2D_SRC: return H->src;
         case W2D_RES: return H->res;
-        case W2D_OM: return H->om;
+        case W2D_OM: return c.om;
         default: return nullptr;
+    }
+}
+
+// Every buffer of field f a set_field writes, so that each holds the one value the reference keeps (the momentum and
+// the mean depth); the velocities' two buffers are the reference's own.
+std::vector<void*> field_all(w2d_handle* H, int f) {
+    switch (f) {
+        case W2D_H: return {H->hb[0], H->hb[1]};
+        case W2D_MX:
+        case W2D_MX1: return {H->mxb[0], H->mxb[1]};
+        case W2D_MY:
+        case W2D_MY1: return {H->myb[0], H->myb[1]};
+        default: {
+            void* p = field_ptr(H, f);
+            if (!p) return {};
+            return {p};
+        }
     }
 }
 
 template <class R>
 StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     StepArgs<R> A{};
-    A.s.h = (R*)H->h[H->hcur];
-    A.s.h_next = (R*)H->h[1 - H->hcur];
-    A.s.d = (R*)H->d;
-    A.s.u = (R*)H->uv[0][H->ucur];
-    A.s.v = (R*)H->uv[1][H->ucur];
-    A.s.mx = (R*)H->uv[2][H->ucur];
-    A.s.my = (R*)H->uv[3][H->ucur];
-    A.s.u1 = (R*)H->uv[0][1 - H->ucur];
-    A.s.v1 = (R*)H->uv[1][1 - H->ucur];
-    A.s.mx1 = (R*)H->uv[2][1 - H->ucur];
-    A.s.my1 = (R*)H->uv[3][1 - H->ucur];
-    A.s.qx = (R*)H->qx;
-    A.s.qy = (R*)H->qy;
-    A.s.qx1 = (R*)H->qx1;
-    A.s.qy1 = (R*)H->qy1;
-    A.s.res = (R*)H->res;
-    A.s.src = (R*)H->src;
-    A.s.om = (R*)H->om;
+    A.b = bufs<R>(H);
+    A.par = H->par;
     A.st.hc = H->hc;
     A.st.ctab = H->ctab;
     A.st.ftabx = H->ftabx;
@@ -2114,7 +2272,6 @@ StepArgs<R> step_args(w2d_handle* H, double dt, int want_wave) {
     A.want_wave = want_wave;
     A.wave = (R*)H->wave;
     A.wet = H->wet;
-    A.h_old = (const R*)H->h[1 - H->hcur];
     A.fresh = H->m_fresh;
     return A;
 }
@@ -2132,41 +2289,41 @@ int mark(w2d_handle* H) {
     return 0;
 }
 
-// The face and cell kernels at the handle's workgroup sizes: 256 threads (4 cells each) or 512 (2 cells each; the
-// same LDS per workgroup, twice the waves per CU).
+// One step's kernels on `blocks` workgroups (a device-resident run launches one per tile; those without a listed tile
+// leave at once): the learned closure when it is on, then the face kernel and the cell kernel at the handle's
+// workgroup sizes (256 or 512 threads). With timing on, marks go before the first kernel, between the two and after.
 template <class R, bool FAST>
-int launch_faces(w2d_handle* H, const StepArgs<R>& A, int blocks) {
-    if (H->closure_n) {
+int launch_step_kernels(w2d_handle* H, const StepArgs<R>& A, int blocks) {
+    int s;
+    if (H->timing && (s = mark(H))) return s;
+    const bool learned = H->closure_n > 0;
+    if (learned) {
         if constexpr (std::is_same<R, float>::value) {
             ClosureArgs C{H->closure_n, H->cl_cell_r, H->cl_cell_l, H->cl_axis, H->cl_stat, H->cl_norm, H->cl_wbf,
                           H->cl_params, H->sx, H->sy, H->corr_x, H->corr_y, H->closure_conveyance, A.members,
                           A.tile_member};
             if (A.members || A.want_wave) {
                 const int cb = std::min((H->closure_n + 63) / 64, 4096);
-                closure_kernel<<<cb, 256, 0, H->stream>>>(C, A.s.d, A.s.h);
+                closure_kernel<<<cb, 256, 0, H->stream>>>(C, A.b, A.par);
                 HIP_TRY(hipGetLastError());
             }
             faces_kernel<float, FAST, 512, true><<<blocks, 512, 0, H->stream>>>(A);
-            HIP_TRY(hipGetLastError());
-            return 0;
         } else {
             return fail(1, "the learned closure runs in FP32");
         }
-    }
-    if (H->faces_threads == 512)
+    } else if (H->faces_threads == 512) {
         faces_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
-    else
+    } else {
         faces_kernel<R, FAST, 256><<<blocks, 256, 0, H->stream>>>(A);
+    }
     HIP_TRY(hipGetLastError());
-    return 0;
-}
-template <class R, bool FAST>
-int launch_cells(w2d_handle* H, const StepArgs<R>& A, int blocks) {
+    if (H->timing && (s = mark(H))) return s;
     if (H->cells_threads == 512)
         cells_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
     else
         cells_kernel<R, FAST, 256><<<blocks, 256, 0, H->stream>>>(A);
     HIP_TRY(hipGetLastError());
+    if (H->timing && (s = mark(H))) return s;
     return 0;
 }
 
@@ -2174,14 +2331,11 @@ template <class R, bool FAST>
 int launch_step(w2d_handle* H, double dt, int want_wave) {
     const StepArgs<R> A = step_args<R>(H, dt, want_wave);
     int s;
-    if (H->timing && (s = mark(H))) return s;
-    if ((s = launch_faces<R, FAST>(H, A, H->nact))) return s;
-    if (H->timing && (s = mark(H))) return s;
-    if ((s = launch_cells<R, FAST>(H, A, H->nact))) return s;
-    if (H->timing && (s = mark(H))) return s;
+    if ((s = launch_step_kernels<R, FAST>(H, A, H->nact))) return s;
     if (H->nb_cells) {
+        const Cur<R> c = view(A.b, H->par);
         boundary_kernel<R><<<(H->nb_cells + 255) / 256, 256, 0, H->stream>>>(
-            H->nb_cells, H->b_cell, H->b_ref, H->b_kind, H->b_d0, H->b_h0, (R*)H->h[1 - H->hcur], (R*)H->d);
+            H->nb_cells, H->b_cell, H->b_ref, H->b_kind, H->b_d0, H->b_h0, c.h_next, c.d);
         HIP_TRY(hipGetLastError());
     }
     return 0;
@@ -2194,23 +2348,29 @@ int materialize(w2d_handle* H) {
     const bool run = H->run_m > 0;
     const int blocks = run ? H->ntiles : H->nact;
     if (blocks > 0) {
-        const int o = 1 - H->ucur;
+        const Cur<char> c = cur(H);
+        const int p2 = H->par & 1;
         if (H->prec == 8)
             momentum_kernel<double><<<blocks, NT, 0, H->stream>>>(
-                H->active, H->nact, run ? H->run_glob : nullptr, H->nbr, (const double*)H->h[1 - H->hcur],
-                (const double*)H->uv[0][H->ucur], (const double*)H->uv[1][H->ucur], (double*)H->uv[2][H->ucur],
-                (double*)H->uv[3][H->ucur], (double*)H->uv[2][o], (double*)H->uv[3][o], H->P.dx);
+                H->active, H->nact, run ? H->run_glob : nullptr, H->nbr, (const double*)c.h_next, (const double*)c.u,
+                (const double*)c.v, (double*)H->mxb[p2], (double*)H->myb[p2], (double*)H->mxb[1 - p2],
+                (double*)H->myb[1 - p2], H->P.dx);
         else
             momentum_kernel<float><<<blocks, NT, 0, H->stream>>>(
-                H->active, H->nact, run ? H->run_glob : nullptr, H->nbr, (const float*)H->h[1 - H->hcur],
-                (const float*)H->uv[0][H->ucur], (const float*)H->uv[1][H->ucur], (float*)H->uv[2][H->ucur],
-                (float*)H->uv[3][H->ucur], (float*)H->uv[2][o], (float*)H->uv[3][o], (float)H->P.dx);
+                H->active, H->nact, run ? H->run_glob : nullptr, H->nbr, (const float*)c.h_next, (const float*)c.u,
+                (const float*)c.v, (float*)H->mxb[p2], (float*)H->myb[p2], (float*)H->mxb[1 - p2],
+                (float*)H->myb[1 - p2], (float)H->P.dx);
         HIP_TRY(hipGetLastError());
     }
     HIP_TRY(hipStreamSynchronize(H->stream));
     H->m_fresh = 1;
     return 0;
 }
+
+// After steps of a device-resident run: applies the list check the last step asked for, brings every member's buffers
+// to one parity (the first member's) and takes it as the handle's, and follows the device's active list. Every call that
+// reads or writes the state through the protocol makes it first (a no-op when no run step was enqueued since).
+int run_align(w2d_handle* H);
 
 // Fields that hold the face momentum, and fields whose change would move the momentum the last step left.
 bool momentum_store(int f) { return f == W2D_MX || f == W2D_MY || f == W2D_MX1 || f == W2D_MY1; }
@@ -2318,12 +2478,10 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     if (hipGetDevice(&H->device) != hipSuccess) return bail(fail(3, "hipGetDevice failed"));
     if ((s = scratch_address(H->device, &H->scratch_dev))) return bail(s);
     const size_t n = H->ncell, es = esize(H);
-    for (int b = 0; b < 2; ++b)
-        if ((s = dalloc(H, &H->h[b], n * es))) return bail(s);
-    for (int f = 0; f < 4; ++f)
+    for (void** pair : {H->hb, H->ub, H->vb, H->mxb, H->myb})
         for (int b = 0; b < 2; ++b)
-            if ((s = dalloc(H, &H->uv[f][b], n * es))) return bail(s);
-    for (void** a : {&H->d, &H->qx, &H->qy, &H->qx1, &H->qy1, &H->res, &H->src, &H->om})
+            if ((s = dalloc(H, &pair[b], n * es))) return bail(s);
+    for (void** a : {&H->d, &H->qx, &H->qy, &H->om, &H->qx1, &H->qy1, &H->res, &H->src})
         if ((s = dalloc(H, a, n * es))) return bail(s);
     if ((s = dalloc(H, &H->wave, (size_t)H->ntiles * es))) return bail(s);
     if ((s = dalloc(H, (void**)&H->wet, (size_t)H->ntiles * sizeof(int)))) return bail(s);
@@ -2410,6 +2568,7 @@ int w2d_set_field(w2d_handle* H, int32_t f, const void* host, int64_t n) {
     void* dst = field_ptr(H, f);
     if (!dst) return fail(1, "set_field: the device keeps no copy of this field");
     int s;
+    if ((s = run_align(H))) return s;
     if (momentum_field(f) && (s = materialize(H))) return s;
     const size_t es = esize(H);
     std::vector<char> own(H->ncell * es);
@@ -2419,14 +2578,8 @@ int w2d_set_field(w2d_handle* H, int32_t f, const void* host, int64_t n) {
         compact(H, (const float*)host, (float*)own.data());
     // the void tile stays zero whatever the host wrote into tile 0
     std::memset(own.data(), 0, TILE2 * es);
-    HIP_TRY(hipMemcpyAsync(dst, own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
-    if (f == W2D_H) HIP_TRY(hipMemcpyAsync(H->h[1 - H->hcur], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
-    // the momentum is one value per face whichever buffer is current (no step stores a second one)
-    if (f == W2D_MX || f == W2D_MY || f == W2D_MX1 || f == W2D_MY1) {
-        const int k = (f == W2D_MX || f == W2D_MX1) ? 2 : 3;
-        HIP_TRY(hipMemcpyAsync(H->uv[k][0], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
-        HIP_TRY(hipMemcpyAsync(H->uv[k][1], own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
-    }
+    // every buffer of a field the reference keeps once (the momentum: one value per face whichever buffer is current)
+    for (void* b : field_all(H, f)) HIP_TRY(hipMemcpyAsync(b, own.data(), own.size(), hipMemcpyHostToDevice, H->stream));
     HIP_TRY(hipStreamSynchronize(H->stream));
     return 0;
 }
@@ -2436,6 +2589,8 @@ int w2d_get_field(w2d_handle* H, int32_t f, void* host, int64_t n) {
     void* src = field_ptr(H, f);
     if (!src) return fail(1, "get_field: the device keeps no copy of this field (the positivity ratio lives in LDS)");
     int s;
+    if ((s = run_align(H))) return s;
+    src = field_ptr(H, f);
     if (momentum_store(f) && (s = materialize(H))) return s;
     const size_t es = esize(H);
     std::vector<char> own(H->ncell * es);
@@ -2453,6 +2608,8 @@ int w2d_read_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, vo
     void* src = field_ptr(H, f);
     if (!src) return fail(1, "read_cells: the device keeps no copy of this field");
     int s;
+    if ((s = run_align(H))) return s;
+    src = field_ptr(H, f);
     if (momentum_store(f) && (s = materialize(H))) return s;
     if (n == 0) return 0;
     const size_t es = esize(H);
@@ -2495,6 +2652,8 @@ int w2d_read_cells_noalloc(w2d_handle* H, int32_t f, const int64_t* index, int64
     if (!src) return fail(1, "read_cells_noalloc: the device keeps no copy of this field");
     if (!H->scratch_dev) return fail(3, "read_cells_noalloc: no scratch address for this handle's device");
     int s;
+    if ((s = run_align(H))) return s;
+    src = field_ptr(H, f);
     if (momentum_store(f) && (s = materialize(H))) return s;
     if (n == 0) return 0;
     const size_t es = esize(H);
@@ -2545,18 +2704,14 @@ int w2d_write_cells(w2d_handle* H, int32_t f, const int64_t* index, int64_t n, c
         val.insert(val.end(), (const char*)values + m * es, (const char*)values + (m + 1) * es);
     }
     int s;
+    if ((s = run_align(H))) return s;
     if (momentum_field(f) && (s = materialize(H))) return s;
     const int64_t k = (int64_t)at.size();
     void* buf = nullptr;
     HIP_TRY(hipMalloc(&buf, k * (sizeof(int64_t) + es)));
     int64_t* d_at = (int64_t*)buf;
     void* d_val = (char*)buf + k * sizeof(int64_t);
-    std::vector<void*> targets{dst};
-    if (f == W2D_H) targets.push_back(H->h[1 - H->hcur]);
-    if (f == W2D_MX || f == W2D_MY || f == W2D_MX1 || f == W2D_MY1) {
-        const int q = (f == W2D_MX || f == W2D_MX1) ? 2 : 3;
-        targets = {H->uv[q][0], H->uv[q][1]};
-    }
+    const std::vector<void*> targets = field_all(H, f);
     hipError_t e = hipMemcpyAsync(d_at, at.data(), k * sizeof(int64_t), hipMemcpyHostToDevice, H->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(d_val, val.data(), k * es, hipMemcpyHostToDevice, H->stream);
     const unsigned blocks = (unsigned)std::min<int64_t>((k + 255) / 256, 1024);
@@ -2582,21 +2737,22 @@ int w2d_set_active(w2d_handle* H, const int32_t* tiles, int32_t n) {
         if (tiles[a] <= 0 || tiles[a] >= H->ntiles) return fail(1, "set_active: tile out of range");
         now[tiles[a]] = 1;
     }
-    // a tile that leaves the list keeps its mean depth in both buffers
+    int s;
+    if ((s = run_align(H))) return s;
+    // a tile that leaves the list keeps one value in every buffer of a field the reference keeps once
     std::vector<int32_t> leaving;
     for (int t : H->active_host)
         if (!now[t]) leaving.push_back(t);
     if (!leaving.empty()) {
         // the momentum of every listed tile is stored first: a leaving tile's depth buffers are about to be made one
-        int s;
         if ((s = materialize(H))) return s;
         HIP_TRY(hipMemcpyAsync(H->tiles_scratch, leaving.data(), leaving.size() * sizeof(int), hipMemcpyHostToDevice, H->stream));
         if (H->prec == 8)
-            copy_tiles_kernel<double><<<(int)leaving.size(), NT, 0, H->stream>>>(H->tiles_scratch, (int)leaving.size(),
-                                                                                (double*)H->h[H->hcur], (double*)H->h[1 - H->hcur]);
+            leave_tiles_kernel<double><<<(int)leaving.size(), NT, 0, H->stream>>>(
+                H->tiles_scratch, (int)leaving.size(), bufs<double>(H), H->par);
         else
-            copy_tiles_kernel<float><<<(int)leaving.size(), NT, 0, H->stream>>>(H->tiles_scratch, (int)leaving.size(),
-                                                                               (float*)H->h[H->hcur], (float*)H->h[1 - H->hcur]);
+            leave_tiles_kernel<float><<<(int)leaving.size(), NT, 0, H->stream>>>(
+                H->tiles_scratch, (int)leaving.size(), bufs<float>(H), H->par);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipStreamSynchronize(H->stream));
     }
@@ -2642,11 +2798,12 @@ int w2d_step(w2d_handle* H, double dt, int32_t want_wave, int32_t want_wet) {
         HIP_TRY(hipGetLastError());
     }
     if (H->nact == 0) return 0;
-    int s = run_step(H, dt, want_wave);
-    if (s) return s;
-    H->hcur = 1 - H->hcur;
-    H->ucur = 1 - H->ucur;
+    int s;
+    if ((s = run_align(H))) return s;
+    if ((s = run_step(H, dt, want_wave))) return s;
+    H->par = 1 - H->par;
     H->m_fresh = 0;
+    if (H->run_m) H->members_stale = 1;
     return 0;
 }
 
@@ -2701,14 +2858,17 @@ int w2d_refresh_plan(w2d_handle* H, const int32_t* tiles, int32_t n) {
         if (tiles[a] <= 0 || tiles[a] >= H->ntiles) return fail(1, "refresh_plan: tile out of range");
     HIP_TRY(hipMemcpyAsync(H->tiles_scratch, tiles, n * sizeof(int), hipMemcpyHostToDevice, H->stream));
     const int last = H->rows - 1;
+    int s;
+    if ((s = run_align(H))) return s;
+    const Cur<char> c = cur(H);
     if (H->prec == 8)
-        plan_kernel<double, false><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (double*)H->h[H->hcur], (double*)H->om, H->hc,
+        plan_kernel<double, false><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (double*)c.h, (double*)c.om, H->hc,
                                                            H->ctab, H->ncell, last);
     else if (H->fast)
-        plan_kernel<float, true><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (float*)H->h[H->hcur], (float*)H->om, H->hc,
+        plan_kernel<float, true><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (float*)c.h, (float*)c.om, H->hc,
                                                          H->ctab, H->ncell, last);
     else
-        plan_kernel<float, false><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (float*)H->h[H->hcur], (float*)H->om, H->hc,
+        plan_kernel<float, false><<<n, NT, 0, H->stream>>>(H->tiles_scratch, n, (float*)c.h, (float*)c.om, H->hc,
                                                           H->ctab, H->ncell, last);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(H->stream));
@@ -2743,12 +2903,15 @@ int w2d_tally_add(w2d_handle* H, double dt) {
     if (!H) return fail(1, "tally_add: no handle");
     if (!H->n_tally) return 0;
     const int n = H->n_tally;
+    int s;
+    if ((s = run_align(H))) return s;
+    const Cur<char> c = cur(H);
     if (H->prec == 8)
         tally_kernel<double><<<(n + 255) / 256, 256, 0, H->stream>>>(n, dt, H->t_face, H->t_axis, H->t_sign,
-                                                                     (double*)H->qx, (double*)H->qy, H->t_total, H->t_flow);
+                                                                     (double*)c.qx, (double*)c.qy, H->t_total, H->t_flow);
     else
         tally_kernel<float><<<(n + 255) / 256, 256, 0, H->stream>>>(n, dt, H->t_face, H->t_axis, H->t_sign,
-                                                                    (float*)H->qx, (float*)H->qy, H->t_total, H->t_flow);
+                                                                    (float*)c.qx, (float*)c.qy, H->t_total, H->t_flow);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -2808,9 +2971,11 @@ int w2d_storage(w2d_handle* H, double* out) {
         *out = 0.0;
         return 0;
     }
-    if (H->prec == 8)
-        return big_sum<double>(H, H->n_store, H->store_index, (double*)H->h[H->hcur], (double*)H->res, out);
-    return big_sum<float>(H, H->n_store, H->store_index, (float*)H->h[H->hcur], (float*)H->res, out);
+    int s;
+    if ((s = run_align(H))) return s;
+    const Cur<char> c = cur(H);
+    if (H->prec == 8) return big_sum<double>(H, H->n_store, H->store_index, (double*)c.h, (double*)H->res, out);
+    return big_sum<float>(H, H->n_store, H->store_index, (float*)c.h, (float*)H->res, out);
 }
 
 int w2d_maxima_set(w2d_handle* H, double arrive, double deep) {
@@ -2833,17 +2998,20 @@ int w2d_maxima_set(w2d_handle* H, double arrive, double deep) {
 
 int w2d_maxima_update(w2d_handle* H, double t) {
     if (!H || !H->maxima[0]) return fail(1, "maxima_update: call maxima_set first");
+    int s;
+    if ((s = run_align(H))) return s;
     if (!H->nact) return 0;
+    const Cur<char> c = cur(H);
     if (H->prec == 8)
         maxima_kernel<double><<<H->nact, NT, 0, H->stream>>>(
-            H->active, H->nact, H->nbr, t, H->m_arrive, H->m_deep, (double*)H->d, (double*)H->h[H->hcur],
-            (double*)H->uv[0][H->ucur], (double*)H->uv[1][H->ucur], (double*)H->maxima[0], (double*)H->maxima[1],
-            (double*)H->maxima[2], (double*)H->maxima[3], (double*)H->maxima[4]);
+            H->active, H->nact, H->nbr, t, H->m_arrive, H->m_deep, (double*)c.d, (double*)c.h, (double*)c.u,
+            (double*)c.v, (double*)H->maxima[0], (double*)H->maxima[1], (double*)H->maxima[2], (double*)H->maxima[3],
+            (double*)H->maxima[4]);
     else
         maxima_kernel<float><<<H->nact, NT, 0, H->stream>>>(
-            H->active, H->nact, H->nbr, (float)t, (float)H->m_arrive, (float)H->m_deep, (float*)H->d, (float*)H->h[H->hcur],
-            (float*)H->uv[0][H->ucur], (float*)H->uv[1][H->ucur], (float*)H->maxima[0], (float*)H->maxima[1],
-            (float*)H->maxima[2], (float*)H->maxima[3], (float*)H->maxima[4]);
+            H->active, H->nact, H->nbr, (float)t, (float)H->m_arrive, (float)H->m_deep, (float*)c.d, (float*)c.h,
+            (float*)c.u, (float*)c.v, (float*)H->maxima[0], (float*)H->maxima[1], (float*)H->maxima[2],
+            (float*)H->maxima[3], (float*)H->maxima[4]);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -3193,11 +3361,9 @@ RunLists run_lists(w2d_handle* H) {
 
 template <class R, bool FAST>
 int enqueue_pre(w2d_handle* H, int flush_only) {
-    R* const* fc = (R* const*)(H->run_faces_ptrs + 8 * H->ucur);  // current u, v, mx, my, then the others
     pre_kernel<R, FAST><<<H->run_m, PRE_THREADS, 0, H->stream>>>(
-        H->run_mem, H->run_glob, H->run_scales, H->run_num, run_lists(H), (R*)H->h[H->hcur], (R*)H->h[1 - H->hcur],
-        (R*)H->om, fc, fc + 4, H->hc, H->ctab, H->ncell, H->rows - 1, (R)H->P.dx, H->m_fresh, H->run_wave,
-        H->run_src_partial, flush_only);
+        H->run_mem, H->run_glob, H->run_scales, H->run_num, run_lists(H), bufs<R>(H), H->hc, H->ctab, H->ncell,
+        H->rows - 1, (R)H->P.dx, H->m_fresh, H->run_wave, H->run_src_partial, flush_only);
     HIP_TRY(hipGetLastError());
     return 0;
 }
@@ -3213,7 +3379,6 @@ int enqueue_run_step(w2d_handle* H) {
     A.member_wave = H->run_wave;
     A.excess = H->excess;
     const int nt = H->ntiles;
-    R* h_new = (R*)H->h[1 - H->hcur];
     int s;
     if (H->excess) {
         excess_kernel<R><<<std::min(nt, 2048), EX_THREADS, 0, H->stream>>>(
@@ -3222,24 +3387,18 @@ int enqueue_run_step(w2d_handle* H) {
         HIP_TRY(hipGetLastError());
     }
     if ((s = enqueue_pre<R, FAST>(H, 0))) return s;
-    if (H->timing && (s = mark(H))) return s;
-    if ((s = launch_faces<R, FAST>(H, A, nt))) return s;
-    if (H->timing && (s = mark(H))) return s;
-    if ((s = launch_cells<R, FAST>(H, A, nt))) return s;
-    if (H->timing && (s = mark(H))) return s;
+    if ((s = launch_step_kernels<R, FAST>(H, A, nt))) return s;
     post_kernel<R><<<H->run_m, POST_THREADS, 0, H->stream>>>(H->run_mem, H->run_wave, H->run_is_active,
                                                      H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
-                                                     H->b_d0, H->b_h0, h_new, (R*)H->d, H->t_face, H->t_axis,
-                                                     H->t_sign, (R*)H->qx, (R*)H->qy, H->t_total, H->t_flow);
+                                                     H->b_d0, H->b_h0, A.b, H->t_face, H->t_axis, H->t_sign,
+                                                     H->t_total, H->t_flow);
     if (H->run_maxima)
         maxima_run_kernel<R><<<nt, NT, 0, H->stream>>>(
-            H->run_glob, H->run_mem, H->run_tile_member, H->active, H->nbr, (R)H->m_arrive, (R)H->m_deep, (R*)H->d,
-            h_new, (R*)H->uv[0][1 - H->ucur], (R*)H->uv[1][1 - H->ucur], (R*)H->maxima[0], (R*)H->maxima[1],
-            (R*)H->maxima[2], (R*)H->maxima[3], (R*)H->maxima[4], H->run_maxima);
+            H->run_glob, H->run_mem, H->run_tile_member, H->active, H->nbr, (R)H->m_arrive, (R)H->m_deep, A.b,
+            (R*)H->maxima[0], (R*)H->maxima[1], (R*)H->maxima[2], (R*)H->maxima[3], (R*)H->maxima[4], H->run_maxima);
     HIP_TRY(hipGetLastError());
-    H->hcur = 1 - H->hcur;
-    H->ucur = 1 - H->ucur;
     H->m_fresh = 0;
+    H->run_dirty = 1;
     return 0;
 }
 
@@ -3255,6 +3414,68 @@ int run_step_any(w2d_handle* H) {
     return enqueue_run_step<float, false>(H);
 }
 
+// The members as the device holds them, brought to the handle's parities where asked (after the handle stepped on
+// its own, its buffers are the current ones).
+int members_read(w2d_handle* H, std::vector<Member>& mem) {
+    mem.resize(H->run_m);
+    HIP_TRY(hipMemcpyAsync(mem.data(), H->run_mem, H->run_m * sizeof(Member), hipMemcpyDeviceToHost, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+int members_write(w2d_handle* H, const std::vector<Member>& mem) {
+    HIP_TRY(hipMemcpyAsync(H->run_mem, mem.data(), H->run_m * sizeof(Member), hipMemcpyHostToDevice, H->stream));
+    HIP_TRY(hipStreamSynchronize(H->stream));
+    return 0;
+}
+
+int run_align(w2d_handle* H) {
+    if (!H->run_m || !H->run_dirty) return 0;
+    int s;
+    if ((s = run_flush(H))) return s;  // a check the last step asked for, as the host loop makes after its step
+    std::vector<Member> mem;
+    if ((s = members_read(H, mem))) return s;
+    RunGlobals g;
+    HIP_TRY(hipMemcpy(&g, H->run_glob, sizeof(g), hipMemcpyDeviceToHost));
+    const int par = mem[0].par;
+    bool moved = false;
+    for (int m = 0; m < H->run_m; ++m) {
+        Member& b = mem[m];
+        if (b.par == par) continue;
+        const int cells = (b.tile1 - b.tile0) * TILE2;
+        const int blocks = std::min((cells + 255) / 256, 4096);
+        if (H->prec == 8)
+            align_kernel<double><<<blocks, 256, 0, H->stream>>>(b.tile0, b.tile1, bufs<double>(H), b.par, par);
+        else
+            align_kernel<float><<<blocks, 256, 0, H->stream>>>(b.tile0, b.tile1, bufs<float>(H), b.par, par);
+        HIP_TRY(hipGetLastError());
+        b.par = par;
+        moved = true;
+    }
+    if (moved && (s = members_write(H, mem))) return s;
+    H->par = par;
+    H->run_dirty = 0;
+    H->members_stale = 0;
+    // the protocol's view of the list follows the device's
+    H->nact = g.nact;
+    H->active_host.resize(g.nact);
+    if (g.nact) HIP_TRY(hipMemcpy(H->active_host.data(), H->active, g.nact * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// Members take the handle's parities when it stepped on its own since they last did (their buffers are the handle's).
+int members_follow(w2d_handle* H) {
+    if (!H->members_stale) return 0;
+    std::vector<Member> mem;
+    int s;
+    if ((s = members_read(H, mem))) return s;
+    for (Member& b : mem) {
+        b.par = H->par;
+    }
+    if ((s = members_write(H, mem))) return s;
+    H->members_stale = 0;
+    return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -3265,6 +3486,7 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         return fail(1, "run_setup: bad arguments");
     const int nt = H->ntiles;
     int s;
+    if ((s = run_align(H))) return s;  // a run set up before ends where it stands
     std::vector<int32_t> tm(nt, 0);
     std::vector<Member> mem(M);
     for (int m = 0; m < M; ++m) {
@@ -3291,6 +3513,7 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         b.dry = d.dry != 0;
         b.dt_min = INFINITY;
         b.t_stop = INFINITY;
+        b.par = H->par;
     }
     // tiles that keep themselves active: a source, a boundary cell (model.py:117-122, 145-150); each member's
     // boundary cells and tally faces must be one run of the lists, in member order
@@ -3383,16 +3606,6 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         H->run_wave = nullptr;
     }
     if ((s = dalloc(H, (void**)&H->run_wave, (size_t)M * sizeof(unsigned long long)))) return s;
-    if (!H->run_faces_ptrs) {
-        // [ucur = 0: current u, v, mx, my, then the others][ucur = 1: the same, swapped]
-        void* ptrs[16];
-        for (int par = 0; par < 2; ++par)
-            for (int f = 0; f < 4; ++f) {
-                ptrs[8 * par + f] = H->uv[f][par];
-                ptrs[8 * par + 4 + f] = H->uv[f][1 - par];
-            }
-        if ((s = upload(H, (void***)&H->run_faces_ptrs, (void**)ptrs, 16))) return s;
-    }
     H->run_num = RunNum{num->cfl, num->dt_max, num->dt_min, num->wave_floor, H->P.dx, H->P.dx * H->P.dx,
                         num->dt_every, num->active_every, H->excess};
     if (H->excess) {
@@ -3426,29 +3639,29 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         H->graph_steps = 0;
     }
     H->run_m = M;
-    H->run_nsteps0 = mem[0].nsteps;
-    H->run_hcur0 = H->hcur;
-    H->run_ucur0 = H->ucur;
+    H->run_dirty = 0;
+    H->members_stale = 0;
     HIP_TRY(hipStreamSynchronize(H->stream));
     return 0;
 }
 
 int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
-    if (!H || !H->run_m || nsteps < 0) return fail(1, "run_steps: call run_setup first");
-    if (graph_steps > 0 && graph_steps % 2) return fail(1, "run_steps: a graph holds an even number of steps");
+    if (!H || !H->run_m || nsteps < 0 || graph_steps < 0) return fail(1, "run_steps: call run_setup first");
     int done = 0, s;
+    if ((s = members_follow(H))) return s;
     // a momentum stored from the host is read by one step launched on its own: captured steps take it as recomputed
     if (H->m_fresh && nsteps > 0) {
         if ((s = run_step_any(H))) return s;
         ++done;
     }
-    // HIP cannot time events recorded inside a captured graph: timed steps are launched one by one
+    // HIP cannot time events recorded inside a captured graph: timed steps are launched one by one. Every kernel of a
+    // step takes its buffers from its member's parities on the device, so one graph replays at any parity.
     if (graph_steps > 0 && !H->timing) {
         if (H->graph && H->graph_steps != graph_steps) {
             (void)hipGraphExecDestroy(H->graph);
             H->graph = nullptr;
         }
-        if (!H->graph && nsteps >= graph_steps) {
+        if (!H->graph && nsteps - done >= graph_steps) {
             hipGraph_t g;
             HIP_TRY(hipStreamBeginCapture(H->stream, hipStreamCaptureModeThreadLocal));
             for (int q = 0; q < graph_steps; ++q)
@@ -3460,17 +3673,12 @@ int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
             HIP_TRY(hipGraphInstantiate(&H->graph, g, nullptr, nullptr, 0));
             (void)hipGraphDestroy(g);
             H->graph_steps = graph_steps;
-            H->graph_parity = H->hcur;
-            // capturing enqueued nothing; the parities advanced graph_steps (even) times and are back
         }
-        if (H->graph && H->hcur != H->graph_parity && done < nsteps) {
-            if ((s = run_step_any(H))) return s;
-            ++done;
-        }
-        while (H->graph && H->hcur == H->graph_parity && nsteps - done >= graph_steps) {
+        while (H->graph && nsteps - done >= graph_steps) {
             HIP_TRY(hipGraphLaunch(H->graph, H->stream));
             done += graph_steps;
         }
+        H->run_dirty = 1;
     }
     for (; done < nsteps; ++done)
         if ((s = run_step_any(H))) return s;
@@ -3490,13 +3698,11 @@ int w2d_run_pause(w2d_handle* H, const double* t_stop, int32_t M) {
 
 int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
     if (!H || !out || M != H->run_m) return fail(1, "run_read: bad arguments");
-    std::vector<Member> mem(M);
-    RunGlobals g;
-    int s = run_flush(H);  // a check the last step asked for is applied now, as the host loop does after its step
-    if (s) return s;
-    HIP_TRY(hipMemcpyAsync(mem.data(), H->run_mem, M * sizeof(Member), hipMemcpyDeviceToHost, H->stream));
-    HIP_TRY(hipMemcpyAsync(&g, H->run_glob, sizeof(g), hipMemcpyDeviceToHost, H->stream));
-    HIP_TRY(hipStreamSynchronize(H->stream));
+    H->run_dirty = 1;  // a check the last step asked for is applied now, as the host loop does after its step
+    int s;
+    if ((s = run_align(H))) return s;
+    std::vector<Member> mem;
+    if ((s = members_read(H, mem))) return s;
     for (int m = 0; m < M; ++m) {
         const Member& b = mem[m];
         w2d_member_state& o = out[m];
@@ -3513,16 +3719,6 @@ int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
         const double share = len > 0.0 ? (std::min(b.t, b.window_end) - b.window_start) / len : 0.0;
         o.rain = b.rain_done + b.rain_window * share;
     }
-    if (M == 1 && !mem[0].done) {
-        const long long taken = mem[0].nsteps - H->run_nsteps0;
-        H->hcur = (int)((H->run_hcur0 + taken) & 1);
-        H->ucur = (int)((H->run_ucur0 + taken) & 1);
-    }
-    // the protocol's view of the list follows the device's
-    H->nact = g.nact;
-    H->active_host.resize(g.nact);
-    if (g.nact)
-        HIP_TRY(hipMemcpy(H->active_host.data(), H->active, g.nact * sizeof(int), hipMemcpyDeviceToHost));
     return 0;
 }
 

@@ -215,15 +215,22 @@ class BatchedResult:
     tuning: list | None = None
 
 
-def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chunk: int = 512, tune: bool = True,
-                launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None, on_chunk=None,
-                prepare=None) -> BatchedResult:  # fmt: skip
-    """Every member in one device-resident run to its end; per-member max depth (sampled every ``maxima_every``
-    steps), mass error."""
-    from wadi2d_amd.engine.autotune import tune_launch
+@dataclass
+class Batch:
+    """An ensemble's mosaic model with its device run set up: the members' tally faces (in member order) and the
+    member of every tile."""
 
+    model: Model
+    run: DeviceRun
+    faces: np.ndarray
+    member_of: np.ndarray
+
+
+def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, prepare=None, **options) -> Batch:
+    """The mosaic model of every member, its boundary cells, source, tally and maxima, and one device run over all
+    members from t = 0 (``options`` go to the backend)."""
     num = ens.scenarios[0].numerics
-    model = Model(ens.tables, num, backend)
+    model = Model(ens.tables, num, backend, **options)
     model.set_depth(np.zeros(ens.tables.grid.shape))
     b = model.backend
     model.set_boundary(ens.bset.cells)
@@ -251,6 +258,20 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
     if prepare is not None:
         prepare(model)
     run = DeviceRun(b, plans, num, maxima=maxima_every, idle=st.idle)
+    return Batch(model, run, faces, member_of)
+
+
+def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chunk: int = 512, tune: bool = True,
+                launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None, on_chunk=None,
+                prepare=None) -> BatchedResult:  # fmt: skip
+    """Every member in one device-resident run to its end; per-member max depth (sampled every ``maxima_every``
+    steps), mass error."""
+    from wadi2d_amd.engine.autotune import tune_launch
+
+    bt = setup_batched(ens, backend, maxima_every, prepare)
+    model, run, faces, member_of = bt.model, bt.run, bt.faces, bt.member_of
+    b = model.backend
+    plans = run.plans
     b.synchronize()
     t0 = time.perf_counter()
     tuning = None
