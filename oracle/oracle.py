@@ -442,12 +442,53 @@ class Balance:
         return self.error / scale
 
 
+class Maxima:
+    """The running maxima of the active tiles' own cells, restating engine/maxima.py:29-51 (``_update_cpu``) statement
+    for statement: level above the bed (depth), mean depth, speed at the cell centre from its west/east and
+    north/south face velocities where the level exceeds ``deep``, speed times level (hazard), and the first time the
+    level reaches ``arrive`` (-1 before). In the backend's precision: float32 arrays and arithmetic for ``prec=4``."""
+
+    NAMES = ("depth", "mean_depth", "speed", "hazard", "arrival")
+
+    def __init__(self, model: OracleModel, arrive: float = 0.1, deep: float = 0.05) -> None:
+        self.model = model
+        self.dtype = np.float32 if model.backend.prec == 4 else np.float64
+        n = model.grid.ntiles * STRIDE2
+        self.arrays = [np.zeros(n, self.dtype) for _ in self.NAMES]
+        self.arrays[4][:] = -1.0
+        self.arrive, self.deep = self.dtype(arrive), self.dtype(deep)
+
+    def update(self, t: float) -> None:
+        b = self.model.backend
+        d, h, u, v = (b.get_field(f).astype(self.dtype) for f in ("d", "h", "u", "v"))
+        dmax, hmax, smax, zmax, tarr = self.arrays
+        active = np.asarray(b.active_tiles, np.int64)
+        jj, ii = np.mgrid[0:TILE, 0:TILE]
+        own = ((jj + HALO) * STRIDE + ii + HALO).ravel()
+        k = (active[:, None] * STRIDE2 + own[None, :]).ravel()
+        dk = d[k]
+        dmax[k] = np.where(dk > dmax[k], dk, dmax[k])
+        hmax[k] = np.where(h[k] > hmax[k], h[k], hmax[k])
+        half = self.dtype(0.5)
+        uc = half * (u[k] + u[k + 1])  # the east face is the next state entry, a halo entry at the tile's edge
+        vc = half * (v[k] + v[k + STRIDE])
+        sp = np.sqrt(uc * uc + vc * vc)
+        wet = dk > self.deep
+        smax[k] = np.where(wet & (sp > smax[k]), sp, smax[k])
+        hz = sp * dk
+        zmax[k] = np.where(wet & (hz > zmax[k]), hz, zmax[k])
+        tarr[k] = np.where((tarr[k] < 0) & (dk >= self.arrive), self.dtype(t), tarr[k])
+
+
 class Ledger:
     """What a run accumulates beside the step: the FP64 boundary tally of every step (maxima.py:149-192), the storage
     of the interior cells with their residuals (monitor.py:214-218), the rain credit of each window
-    (monitor.py:197-212), the per-cell maximum depth sampled every step and the peak outfall discharge."""
+    (monitor.py:197-212), the per-cell maximum depth sampled every step, the peak outfall discharge, and with
+    ``maxima_every`` the five running maxima sampled after every step whose count is a multiple of it
+    (maxima.py:103-146 as the run loop calls it)."""
 
-    def __init__(self, model: OracleModel, interior: np.ndarray, xfaces, xsign, yfaces, ysign) -> None:
+    def __init__(self, model: OracleModel, interior: np.ndarray, xfaces, xsign, yfaces, ysign,
+                 maxima_every: int = 0, arrive: float = 0.1, deep: float = 0.05) -> None:  # fmt: skip
         self.model = model
         grid = model.grid
         self.storage_index = np.asarray(state_index(grid, *np.nonzero(interior)), np.int64)
@@ -460,6 +501,8 @@ class Ledger:
         self.last_t = model.stepper.t
         self.dmax = np.zeros(grid.ntiles * STRIDE2)
         self.peak_q = 0.0
+        self.maxima_every = int(maxima_every)
+        self.maxima = Maxima(model, arrive, deep) if self.maxima_every else None
         self.balance.storage0 = self.storage()
 
     def storage(self) -> float:
@@ -487,6 +530,8 @@ class Ledger:
         self.total += dt * q
         self.peak_q = max(self.peak_q, float(q.sum()))
         np.maximum(self.dmax, b.get_field("d"), out=self.dmax)
+        if self.maxima is not None and st.nsteps % self.maxima_every == 0:
+            self.maxima.update(st.t)
 
     def accounts(self, t: float) -> Balance:
         self.settle(t)

@@ -293,8 +293,9 @@ def test_the_command_line_returns_the_protocol_codes(job_folder):
 # ---- runs on the device
 
 
-def _oracle_run(cfg):
-    """The same job through the oracle, in the window loop of run._run (run.py:1712-1739)."""
+def _oracle_run(cfg, full: bool = False):
+    """The same job through the oracle, in the window loop of run._run (run.py:1712-1739); with ``full`` its five
+    running maxima sampled every ``outputs.maxima_every`` steps are returned too."""
     from oracle.oracle import Ledger, OracleModel
     from oracle.oracle import Numerics as ONumerics
     from wadi2d_amd.engine.forcing import boundary_set
@@ -308,7 +309,9 @@ def _oracle_run(cfg):
     m.set_depth(initial_depth(cfg, tables))
     if bset.size:
         m.set_boundary(bset.cells)
-    ledger = Ledger(m, tables.grid.mask & ~bset.mask, bset.xfaces, bset.xsign, bset.yfaces, bset.ysign)
+    ledger = Ledger(m, tables.grid.mask & ~bset.mask, bset.xfaces, bset.xsign, bset.yfaces, bset.ysign,
+                    maxima_every=cfg.outputs.maxima_every if full else 0, arrive=cfg.outputs.arrival_depth,
+                    deep=cfg.outputs.speed_depth)  # fmt: skip
     t, dry, end = 0.0, False, cfg.run.duration_s
     while t < end - 1e-9:
         t_next = min(t + cfg.run.source_every_s, end)
@@ -320,7 +323,48 @@ def _oracle_run(cfg):
         m.advance(t_next, ledger.on_step)
         ledger.settle(t_next)
         t = t_next
+    if full:
+        maxima = {n: from_state(tables.grid, a) for n, a in zip(ledger.maxima.NAMES, ledger.maxima.arrays, strict=True)}
+        return maxima, ledger.accounts(t), m.stepper.nsteps
     return from_state(tables.grid, ledger.dmax), ledger.accounts(t), m.stepper.nsteps
+
+
+def _kinds_job(folder: Path) -> dict:
+    """The test job with a stage and a normal boundary beside the free one and a hyetograph storm."""
+    xe, yn = X0 + NX * DX, Y0 + NY * DX
+    d = job_dict(duration=600.0)
+    d["storm"] = {"kind": "hyetograph", "hyetograph_min": [2.0, 4.0, 12.0], "hyetograph_mm": [10.0, 20.0, 25.0]}
+    d["boundaries"] = [
+        {"kind": "free", "coordinates": [[xe - 1.0, yn], [xe - 1.0, Y0 + 320.0]], "name": "east top"},
+        {"kind": "normal", "slope": 0.01, "coordinates": [[xe - 1.0, Y0 + 310.0], [xe - 1.0, Y0]], "name": "east"},
+        {"kind": "stage", "stage": 100.5, "coordinates": [[X0 + 1.0, Y0 + 100.0], [X0 + 1.0, Y0]], "name": "west"},
+    ]
+    d["outputs"]["maxima_every"] = 4
+    (folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    return d
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("loop", ["host", "device"])
+def test_stage_and_normal_boundaries_and_every_maximum_equal_the_oracle_bit_for_bit(job_folder, loop):
+    """Boundary kinds 0 (stage) and 2 (normal, the inner cell's level lowered by S dx; forcing.py:299-332,
+    cells.cu:95-111) beside a free one, through the exact float32 HIP path and the oracle's float32 statement order;
+    the five running maxima (maxima.py:29-51), sampled every 4 steps as a job does, equal bit for bit."""
+    from wadi2d_amd.run import MAXIMA, run_job
+
+    _kinds_job(job_folder)
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "kinds", backend="hip-exact", loop=loop)
+    assert s.exit_code == 0, s.message
+    want, bal, steps = _oracle_run(cfg, full=True)
+    print(f"{loop}: {s.steps} steps (oracle {steps}), mass {s.balance.relative:.2e} / {bal.relative:.2e}", flush=True)
+    assert s.steps == steps
+    for name in MAXIMA:
+        got = np.load(job_folder / "kinds" / "results" / "maxima" / f"{name}.npy")
+        ref = want[name].astype(np.float32)
+        assert np.array_equal(got, ref), f"{name}: {np.abs(got - ref).max()}"
+    assert (want["speed"] > 0.1).sum() > 100 and (want["arrival"] > 0).sum() > 100  # the maxima are not trivial
+    assert abs(s.balance.relative - bal.relative) <= 1e-9
 
 
 @pytest.mark.gpu
