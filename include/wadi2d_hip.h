@@ -84,7 +84,8 @@ int w2d_write_cells(w2d_handle* h, int32_t field, const int64_t* index, int64_t 
 
 /* set_active (gpu/backend.py:116-119). */
 int w2d_set_active(w2d_handle* h, const int32_t* tiles, int32_t n);
-/* set_boundary (gpu/backend.py:127-132), BoundaryCells (solver/backend.py:41-55) at state indices. */
+/* set_boundary (gpu/backend.py:127-132), BoundaryCells (solver/backend.py:41-55) at state indices; a cell's ref is read only
+ * for kind 2 (it follows that cell), so kinds 0 and 1 may carry any index, as the reference's callers pass. */
 int w2d_set_boundary(w2d_handle* h, const int32_t* cell, const int32_t* ref, const int32_t* kind, const float* d0,
                      const float* h0, int32_t n);
 /* step (gpu/backend.py:151-188): faces, positivity, update, boundary; then the face buffers swap. */

@@ -324,6 +324,7 @@ def _oracle_run(cfg, full: bool = False):
         ledger.settle(t_next)
         t = t_next
     if full:
+        ledger.maxima.update(t)  # the final sample a run takes as it finishes (monitor.py:298-300)
         maxima = {n: from_state(tables.grid, a) for n, a in zip(ledger.maxima.NAMES, ledger.maxima.arrays, strict=True)}
         return maxima, ledger.accounts(t), m.stepper.nsteps
     return from_state(tables.grid, ledger.dmax), ledger.accounts(t), m.stepper.nsteps

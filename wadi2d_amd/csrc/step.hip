@@ -2769,8 +2769,12 @@ int w2d_set_boundary(w2d_handle* H, const int32_t* cell, const int32_t* ref, con
     std::vector<int32_t> c(n), r(n);
     for (int m = 0; m < n; ++m) {
         const int64_t a = own_of(H, cell[m]);
-        const int64_t b = own_of(H, ref[m]);
-        if (a < 0 || b < 0) return fail(1, "set_boundary: a boundary or reference cell is not an own cell");
+        if (a < 0) return fail(1, "set_boundary: a boundary cell is not an own cell");
+        if (kind[m] < 0 || kind[m] > 2) return fail(1, "set_boundary: a kind is not 0, 1 or 2");
+        // only a following cell (kind 2) reads its reference; the others may carry any index, as the reference's
+        // callers pass (a free outflow's ref is 0 there)
+        const int64_t b = kind[m] == 2 ? own_of(H, ref[m]) : a;
+        if (b < 0) return fail(1, "set_boundary: a following cell's reference is not an own cell");
         c[m] = (int32_t)a;
         r[m] = (int32_t)b;
     }
