@@ -356,16 +356,6 @@ def precision(args) -> int:
     return 0
 
 
-def _held_out_member(m: int):
-    from wadi2d_amd.closure.data import CoarseFaces, draw_storm, member_pair
-
-    storm = draw_storm(np.random.default_rng(9000 + m))
-    _, c = member_pair(9000 + m, 1024, storm, 3 * 3600.0)
-    cf = CoarseFaces.of(c)
-    c.bed = None
-    return c, cf
-
-
 def learned(args) -> int:
     """The learned closure inside the batched launch: held-out 1024 x 1024 members at 20 m (crested ground, 2 m
     pixels), run together with and without the closure, and one after another the plain way; and its gain on
@@ -375,18 +365,12 @@ def learned(args) -> int:
     from wadi2d_amd.closure.model import ClosureModel
     from wadi2d_amd.engine.ensemble import Ensemble, run_serial
 
-    from concurrent.futures import ProcessPoolExecutor
+    from wadi2d_amd.closure.data import held_out_members
 
     model = ClosureModel.load(os.path.join(ROOT, "wadi2d_amd", "closure", "trained.npz"))
-    t0 = time.time()
-    coarse, faces = [], []
-    # the members' ground, tables and face statistics on the host, a few at a time (before the GPU is touched)
-    with ProcessPoolExecutor(max_workers=max(1, args.workers)) as pool:
-        for m, (c, cf) in enumerate(pool.map(_held_out_member, range(args.held_out))):
-            coarse.append(c)
-            faces.append(cf)
-            log(f"[bench] member {m}: {cf.closure_faces(1).sum() + cf.closure_faces(0).sum()} closure faces, "
-                f"{time.time() - t0:.0f} s")  # fmt: skip
+    # the members' ground, tables and face statistics on the host, in worker processes
+    pairs = held_out_members(args.held_out, args.workers, log=log)
+    coarse, faces = [c for c, _ in pairs], [f for _, f in pairs]
     plain, _ = coarse_run(coarse, faces, None, graph_steps=args.graph)
     log(f"[bench] batched plain: {plain.wall_s:.1f} s")
     got, _ = coarse_run(coarse, faces, model, graph_steps=args.graph)

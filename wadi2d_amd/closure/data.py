@@ -17,6 +17,8 @@ side, at one snapshot: the 16 features (closure/features.py) and two targets wit
 
 from __future__ import annotations
 
+import os
+
 from dataclasses import dataclass
 
 import numpy as np
@@ -43,6 +45,38 @@ def member_pair(seed: int, n: int, storm, duration_s: float):
 
 def draw_storm(rng) -> ScsStorm:
     return ScsStorm(float(rng.uniform(40.0, 120.0)), float(rng.uniform(1.0, 3.0)), 1.0)
+
+
+def held_out_member(m: int, n: int = 1024, duration_s: float = 3 * 3600.0):
+    """Held-out member m of the closure's evaluation (seed 9000 + m, never trained on): the coarse n x n crested
+    scenario at 20 m from 2 m pixels with its drawn storm, and its face statistics; the pixels are dropped after."""
+    storm = draw_storm(np.random.default_rng(9000 + m))
+    c = synth.crest_case(n, COARSE_DX, RATIO, 9000 + m, storm, duration_s, n * COARSE_DX, f"coarse {9000 + m}")
+    cf = CoarseFaces.of(c)
+    c.bed = None
+    return c, cf
+
+
+def _one_thread() -> None:
+    os.environ["OMP_NUM_THREADS"] = "1"
+
+
+def held_out_members(count: int, workers: int, log=None) -> list:
+    """Members 0 .. count-1 built in fresh processes (spawned, one thread each: a forked child of a process whose
+    OpenMP pool is running can deadlock), in member order."""
+    import multiprocessing
+    import time
+    from concurrent.futures import ProcessPoolExecutor
+
+    t0 = time.time()
+    out = []
+    ctx = multiprocessing.get_context("spawn")
+    with ProcessPoolExecutor(max_workers=max(1, workers), mp_context=ctx, initializer=_one_thread) as pool:
+        for m, pair in enumerate(pool.map(held_out_member, range(count))):
+            out.append(pair)
+            if log is not None:
+                log(f"[closure] held-out member {m} built, {time.time() - t0:.0f} s")
+    return out
 
 
 @dataclass
