@@ -1,10 +1,14 @@
-"""Rain that varies by cell and by time (wadi2d/storm/rain_frames.py:74-185, storm/gridded.py:73-101,
-hydrology/cell_rain.py:19-58): frames of rain depth on a pixel grid; each 2D cell takes the pixel its centre falls in,
-and a pixel's rain accumulates linearly through each frame's period and stays level across a gap.
+"""Rain that varies over the ground and in time: a stack of rain grids, one grid per period, sampled per 2D cell.
 
-The reference reads NetCDF, DSS and GeoTIFF stacks (io/rain/); none of their libraries is in this image, so frames
-come as ``.npz``: ``depths`` (frames, rows, columns) in mm per frame, NaN for no data; ``starts_min`` and
-``ends_min`` (one per frame); ``west``, ``north`` (the grid's outer corner), ``dx``, ``dy`` (pixel size, m).
+Behaviour followed: wadi2d/storm/rain_frames.py:74-185 (the frames, a point's pixel, per-pixel totals and the
+per-pixel cumulative curve), storm/gridded.py:73-101 (the area reduction) and hydrology/cell_rain.py:19-58 (each cell
+takes the pixel its centre falls in). A pixel's cumulative depth rises at a constant rate through each period and
+holds still through any gap between periods; a NaN in a grid is "no data" and adds nothing.
+
+Files: the reference reads NetCDF, DSS and GeoTIFF stacks through libraries this image does not have, so here a rain
+file is an ``.npz`` holding ``depths`` (periods, rows, columns; mm fallen in each period, NaN for no data),
+``starts_min`` and ``ends_min`` (each period's bounds, minutes), ``west`` and ``north`` (the outer corner of pixel
+(0, 0)) and ``dx``, ``dy`` (pixel width and height, m). Row 0 is the northern row, column 0 the western column.
 """
 
 from __future__ import annotations
@@ -14,11 +18,31 @@ from pathlib import Path
 
 import numpy as np
 
+NPZ_KEYS = ("depths", "starts_min", "ends_min", "west", "north", "dx", "dy")
+
+
+def _problem(depths: np.ndarray, starts: np.ndarray, ends: np.ndarray, dx: float, dy: float) -> str | None:
+    """What is wrong with a stack of rain grids, or None when nothing is."""
+    if depths.ndim != 3:
+        return f"rain depths need three axes (period, row, column), not {depths.ndim}"
+    n = depths.shape[0]
+    if n == 0:
+        return "rain holds no period"
+    if starts.size != n or ends.size != n:
+        return f"{n} rain periods but {starts.size} starts and {ends.size} ends"
+    if not np.all(ends > starts):
+        return "a rain period ends at or before its start"
+    if not np.all(starts[1:] >= ends[:-1]):
+        return "rain periods must come in time order without overlapping"
+    if not (dx > 0.0 and dy > 0.0):
+        return f"rain pixels need a positive size, not {dx} x {dy}"
+    return None
+
 
 @dataclass(frozen=True, eq=False)
 class RainFrames:
-    """``depths`` is (frames, rows, columns) in mm; ``west`` and ``north`` are the outer corner of the grid
-    (rain_frames.py:87-117)."""
+    """A stack of rain grids (the fields of rain_frames.py:87-98): ``depths`` (periods, rows, columns) in mm per period
+    over [``starts_min``, ``ends_min``); pixel (0, 0)'s outer corner at (``west``, ``north``); pixels ``dx`` by ``dy``."""
 
     depths: np.ndarray
     starts_min: np.ndarray
@@ -29,77 +53,82 @@ class RainFrames:
     dy: float
 
     def __post_init__(self) -> None:
-        if self.depths.ndim != 3 or self.depths.shape[0] == 0:
-            raise ValueError("rain frames must be (frames, rows, columns), with at least one frame")
-        if self.starts_min.size != self.depths.shape[0] or self.ends_min.size != self.depths.shape[0]:
-            raise ValueError("rain frames need one start and one end per frame")
-        if np.any(self.ends_min <= self.starts_min) or np.any(self.starts_min[1:] < self.ends_min[:-1]):
-            raise ValueError("rain periods must run forward and must not overlap")
-        if self.dx <= 0.0 or self.dy <= 0.0:
-            raise ValueError("rain pixels must have a size")
+        why = _problem(self.depths, self.starts_min, self.ends_min, self.dx, self.dy)
+        if why:
+            raise ValueError(why)
 
     @property
     def count(self) -> int:
-        return self.depths.shape[0]
-
-    @property
-    def columns(self) -> int:
-        return self.depths.shape[2]
+        """Number of periods."""
+        return int(self.depths.shape[0])
 
     @property
     def rows(self) -> int:
-        return self.depths.shape[1]
+        return int(self.depths.shape[1])
 
-    def pixel_of(self, x: np.ndarray, y: np.ndarray) -> np.ndarray:
-        """Flat index of the pixel each point falls in, -1 off the grid (rain_frames.py:119-125)."""
-        column = np.floor((np.asarray(x) - self.west) / self.dx)
-        row = np.floor((self.north - np.asarray(y)) / self.dy)
-        inside = (column >= 0) & (column < self.columns) & (row >= 0) & (row < self.rows)
-        flat = row * self.columns + column
-        return np.where(inside, np.nan_to_num(flat), -1.0).astype(np.int64)
+    @property
+    def columns(self) -> int:
+        return int(self.depths.shape[2])
+
+    def pixel_of(self, x, y) -> np.ndarray:
+        """Row-major pixel number under each point (x, y); -1 for a point beyond the grid's edges
+        (rain_frames.py:119-125)."""
+        fx = (np.asarray(x, np.float64) - self.west) / self.dx
+        fy = (self.north - np.asarray(y, np.float64)) / self.dy
+        ok = (fx >= 0.0) & (fx < self.columns) & (fy >= 0.0) & (fy < self.rows)  # False for NaN too
+        col = np.floor(np.where(ok, fx, 0.0)).astype(np.int64)
+        row = np.floor(np.where(ok, fy, 0.0)).astype(np.int64)
+        return np.where(ok, row * self.columns + col, -1)
 
     def totals(self) -> np.ndarray:
-        """Depth over the whole event per pixel; NaN where a pixel never had data (rain_frames.py:127-130)."""
-        seen = np.isfinite(self.depths).any(axis=0)
-        return np.where(seen, np.nansum(self.depths, axis=0), np.nan)
+        """(rows, columns): each pixel's depth summed over every period it has data for; NaN for a pixel with data in
+        no period (rain_frames.py:127-130)."""
+        has = ~np.isnan(self.depths)
+        total = np.where(has, self.depths, 0.0).sum(axis=0)
+        return np.where(has.any(axis=0), total, np.nan)
 
 
 class PixelSeries:
-    """Cumulative rain of some pixels against time, linear inside a period and level across a gap between periods
-    (rain_frames.py:167-185)."""
+    """The cumulative depth of chosen pixels as a piecewise-linear function of time (rain_frames.py:167-185).
+
+    The curve runs through two knots per period: at its start the depth of the periods before it, at its end that
+    plus its own. Between one period's end and the next one's start both knots hold the same depth, so the curve is
+    level there; before the first start it is 0, after the last end the event's total.
+    """
 
     def __init__(self, frames: RainFrames, pixels: np.ndarray) -> None:
-        self.edges = np.unique(np.concatenate((frames.starts_min, frames.ends_min)))
-        steps = np.zeros((self.edges.size - 1, pixels.size))
-        at = np.searchsorted(self.edges, frames.starts_min)
-        steps[at] = np.nan_to_num(frames.depths.reshape(frames.count, -1)[:, pixels])
-        self.cumulative = np.vstack((np.zeros(pixels.size), np.cumsum(steps, axis=0)))
+        per_period = np.nan_to_num(frames.depths.reshape(frames.count, -1)[:, np.asarray(pixels)])
+        after = np.cumsum(per_period, axis=0)  # depth once each period has ended
+        before = np.concatenate((np.zeros((1, after.shape[1])), after[:-1]))
+        self.times = np.column_stack((frames.starts_min, frames.ends_min)).ravel()  # start 0, end 0, start 1, ...
+        self.depths = np.stack((before, after), axis=1).reshape(2 * frames.count, -1)
 
     def at(self, minutes: float) -> np.ndarray:
-        """Depth fallen by this time, one value per pixel."""
-        if minutes <= self.edges[0]:
-            return self.cumulative[0]
-        if minutes >= self.edges[-1]:
-            return self.cumulative[-1]
-        k = int(np.searchsorted(self.edges, minutes, side="right")) - 1
-        share = (minutes - self.edges[k]) / (self.edges[k + 1] - self.edges[k])
-        return self.cumulative[k] + (self.cumulative[k + 1] - self.cumulative[k]) * share
+        """Depth fallen by ``minutes``, one value per chosen pixel."""
+        t = self.times
+        if minutes <= t[0]:
+            return self.depths[0]
+        if minutes >= t[-1]:
+            return self.depths[-1]
+        k = int(np.searchsorted(t, minutes, side="right")) - 1  # t[k] <= minutes < t[k + 1]
+        lo, hi = self.depths[k], self.depths[k + 1]
+        return lo + (hi - lo) * ((minutes - t[k]) / (t[k + 1] - t[k]))
 
 
 def read_frames(path: str | Path) -> RainFrames:
-    """Rain frames from an ``.npz`` file (the keys in this module's docstring)."""
+    """A stack of rain grids from an ``.npz`` file holding the arrays named in ``NPZ_KEYS``."""
     p = Path(path)
     if not p.is_file():
-        raise FileNotFoundError(f"rain file not found: {p}")
+        raise FileNotFoundError(f"no rain file at {p}")
     if p.suffix.lower() != ".npz":
-        raise ValueError(f"{p.name}: this path reads rain frames as .npz, not '{p.suffix}' (no NetCDF/DSS/GDAL here)")
+        raise ValueError(f"{p.name}: rain is read from .npz files only (NetCDF, DSS and GeoTIFF readers are absent)")
     with np.load(p, allow_pickle=False) as f:
-        for key in ("depths", "starts_min", "ends_min", "west", "north", "dx", "dy"):
-            if key not in f:
-                raise ValueError(f"{p.name}: no '{key}' array")
-        return RainFrames(np.asarray(f["depths"], np.float64), np.asarray(f["starts_min"], np.float64).ravel(),
-                          np.asarray(f["ends_min"], np.float64).ravel(), float(f["west"]), float(f["north"]),
-                          float(f["dx"]), float(f["dy"]))  # fmt: skip
+        absent = [k for k in NPZ_KEYS if k not in f]
+        if absent:
+            raise ValueError(f"{p.name} lacks {', '.join(absent)}")
+        depths, starts, ends = (np.asarray(f[k], np.float64) for k in NPZ_KEYS[:3])
+        west, north, dx, dy = (float(f[k]) for k in NPZ_KEYS[3:])
+    return RainFrames(depths, starts.ravel(), ends.ravel(), west, north, dx, dy)
 
 
 class CellPixels:

@@ -1,8 +1,11 @@
-"""Rainfall losses (wadi2d/hydrology/losses.py:1-77): the SCS curve-number method in HMS's cumulative-difference
-form, and no loss.
+"""Rain lost before it runs off (behaviour of wadi2d/hydrology/losses.py:1-77).
 
-The device forms the same numbers per cell (``excess_kernel`` in csrc/step.hip): the statements below are the
-ones both follow, in float64.
+Two methods: none (all rain runs off) and the SCS curve number in its cumulative form, in which a period's runoff is
+the rise over the period of the runoff the curve gives for the rain fallen so far. For a curve number CN the
+retention is S = (1000 / CN - 10) inch, the initial abstraction Ia = 0.2 S unless given, and rain P fallen so far
+yields (P - Ia)^2 / (P - Ia + S) of runoff once P passes Ia, none before. An impervious share passes its rain whole.
+
+``excess_kernel`` in csrc/step.hip forms the per-cell numbers on the device with the same float64 statements.
 """
 
 from __future__ import annotations
@@ -13,14 +16,14 @@ import numpy as np
 
 MM_PER_INCH = 25.4  # hydrology/constants.py
 PERCENT = 100.0
-CN_NUMERATOR = 1000.0  # losses.py:18-20
+CN_NUMERATOR = 1000.0  # S in inches is 1000 / CN - 10 (losses.py:18-20)
 CN_OFFSET = 10.0
 DEFAULT_IA_RATIO = 0.2
 
 
 @dataclass(frozen=True)
 class NoLoss:
-    """All rainfall is excess (losses.py:37-44)."""
+    """Every millimetre of rain runs off (losses.py:37-44)."""
 
     def excess(self, precipitation_mm: np.ndarray) -> np.ndarray:
         return np.array(precipitation_mm, dtype=np.float64, copy=True)
@@ -28,8 +31,9 @@ class NoLoss:
 
 @dataclass(frozen=True)
 class ScsCurveNumber:
-    """SCS curve number (losses.py:47-77); ``curve_number`` may be an array (one loss per entry).
-    ``initial_abstraction_mm`` None means 0.2 S."""
+    """The SCS curve-number loss (losses.py:47-77). ``curve_number`` may be one number or an array of them (one
+    independent loss per entry); ``initial_abstraction_mm`` is a depth, or None for 0.2 S; ``percent_impervious``
+    is the share of rain that passes whole."""
 
     curve_number: float | np.ndarray
     initial_abstraction_mm: float | None = None
@@ -37,27 +41,28 @@ class ScsCurveNumber:
 
     @property
     def retention_mm(self):
-        """Potential maximum retention S = (1000 / CN - 10) 25.4 mm."""
-        return (CN_NUMERATOR / np.asarray(self.curve_number, np.float64) - CN_OFFSET) * MM_PER_INCH
+        """S in mm."""
+        inches = CN_NUMERATOR / np.asarray(self.curve_number, np.float64) - CN_OFFSET
+        return inches * MM_PER_INCH
 
     @property
     def abstraction_mm(self):
-        """Initial abstraction Ia, mm."""
-        if self.initial_abstraction_mm is None:
-            return DEFAULT_IA_RATIO * self.retention_mm
-        return self.initial_abstraction_mm
+        """Ia in mm: the given depth, else 0.2 S."""
+        given = self.initial_abstraction_mm
+        return DEFAULT_IA_RATIO * self.retention_mm if given is None else given
 
     def cumulative_excess(self, cumulative_mm) -> np.ndarray:
-        """Pervious excess Pe(P) = (P - Ia)^2 / (P - Ia + S), zero up to Ia."""
-        effective = np.maximum(np.asarray(cumulative_mm, np.float64) - self.abstraction_mm, 0.0)
-        return effective**2 / (effective + self.retention_mm)
+        """Runoff (mm) from the pervious part once ``cumulative_mm`` of rain has fallen."""
+        over = np.maximum(np.asarray(cumulative_mm, np.float64) - self.abstraction_mm, 0.0)
+        return over**2 / (over + self.retention_mm)
 
     def excess(self, precipitation_mm: np.ndarray) -> np.ndarray:
-        """The excess of each period: the difference of cumulative excess, impervious rain passing whole."""
-        p = np.asarray(precipitation_mm, np.float64)
-        pervious = np.diff(self.cumulative_excess(np.cumsum(p)), prepend=0.0)
-        impervious = self.percent_impervious / PERCENT
-        return (1.0 - impervious) * pervious + impervious * p
+        """Runoff of each period from the rain of each period: the pervious part's rise in cumulative runoff, plus the
+        impervious part's rain."""
+        rain = np.asarray(precipitation_mm, np.float64)
+        pervious = np.diff(self.cumulative_excess(np.cumsum(rain)), prepend=0.0)
+        share = self.percent_impervious / PERCENT
+        return (1.0 - share) * pervious + share * rain
 
 
 def cell_excess(depth_mm, curve_number: np.ndarray) -> np.ndarray:

@@ -332,7 +332,9 @@ class Monitor:
         st.idle[:] = saved["idle"]
         st.keep_active[:] = saved["keep_active"]
         b.set_active(saved["active"])
-        b.set_wet(self.model._tile_wet(self.model.field("h")))
+        from wadi2d_amd.solver.model import tiles_touched
+
+        b.set_wet(tiles_touched(self.model.grid, self.model.field("h") > 0.0).astype(np.int32))
         self.next_series, self.next_map = float(saved["next_series"]), float(saved["next_map"])
         self.dt_min = float(saved["dt_min"])
         self.times = list(saved["times"])

@@ -1,4 +1,9 @@
-"""The compute interface a backend implements, and the choice of one (wadi2d/solver/backend.py:1-75)."""
+"""What a step backend is handed, and which backend a name selects (interface of wadi2d/solver/backend.py:1-75).
+
+Two records cross into the backend: ``NumericsRecord``, the scalar constants every kernel reads, and
+``BoundaryCells``, the prescribed-level cells. Their field names and meanings are the reference's, since its callers
+build them; the backend behind the names is this package's HIP library only.
+"""
 
 from __future__ import annotations
 
@@ -11,7 +16,15 @@ from wadi2d_amd.config import G, Numerics
 
 @dataclass(frozen=True)
 class NumericsRecord:
-    """The run-time numbers kernels read; built once from the configuration (backend.py:14-38)."""
+    """Scalar constants of a run, fixed before the first step (backend.py:14-38).
+
+    ``g`` gravity; ``dx`` cell size; ``e_wet`` the depth below which a face carries nothing; ``h_cfl`` the depth
+    below which a cell is left out of the time-step bound; ``u_max`` the speed cap; ``lead`` how many steps one
+    time-step estimate covers (it scales a face's recent speed change in its wave-speed bound); ``a_min`` the floor on
+    a cell's wet share in that bound; ``second`` whether the limited
+    second-order reconstruction is on; ``sheet``, ``mirror`` and ``carry`` the sheet-flow, dry-face mirror and
+    face-depth carry options.
+    """
 
     g: float
     dx: float
@@ -27,14 +40,19 @@ class NumericsRecord:
 
     @classmethod
     def from_config(cls, num: Numerics, dx: float) -> NumericsRecord:
-        lead = float(max(num.dt_every, 1 << num.lts_levels) if num.local_time_stepping else num.dt_every)
-        return cls(G, dx, num.e_wet, num.h_cfl, num.u_max, lead, num.plan_min, num.limiter != "none",
-                   num.sheet_flow, num.mirror_dry_faces, num.face_depth_carry)  # fmt: skip
+        # with local time stepping one estimate must outlast the coarsest level's 2**levels sub-steps
+        lead = num.dt_every
+        if num.local_time_stepping:
+            lead = max(lead, 2**num.lts_levels)
+        return cls(g=G, dx=dx, e_wet=num.e_wet, h_cfl=num.h_cfl, u_max=num.u_max, lead=float(lead),
+                   a_min=num.plan_min, second=num.limiter != "none", sheet=num.sheet_flow,
+                   mirror=num.mirror_dry_faces, carry=num.face_depth_carry)  # fmt: skip
 
 
 @dataclass
 class BoundaryCells:
-    """Cells whose level is prescribed after every update. kind: 0 fixed, 1 free, 2 follows ``ref`` plus d0."""
+    """Cells whose level is set again after each update: ``cell`` their state indices; ``kind`` 0 holds the level at
+    ``h0``, 1 leaves the cell free, 2 sets the depth to the value at cell ``ref`` plus ``d0``."""
 
     cell: np.ndarray
     ref: np.ndarray
@@ -44,9 +62,8 @@ class BoundaryCells:
 
     @classmethod
     def empty(cls) -> BoundaryCells:
-        i = np.zeros(0, np.int32)
-        f = np.zeros(0, np.float32)
-        return cls(i, i.copy(), i.copy(), f, f.copy())
+        none_i, none_f = np.zeros(0, np.int32), np.zeros(0, np.float32)
+        return cls(none_i, none_i.copy(), none_i.copy(), none_f, none_f.copy())
 
 
 BACKENDS = {
@@ -54,9 +71,9 @@ BACKENDS = {
     "hip-exact": ("f32", "exact"),
     "hip-f64": ("f64", "exact"),
 }
-"""Backend names this package answers: float32 with the hardware's fast reciprocal/sqrt (the reference GPU's
-``--use_fast_math`` counterpart), float32 with IEEE division/sqrt (bit for bit the float32 statement order of the
-reference's CPU twin), and float64."""
+"""Backend names and their arithmetic: ``hip`` float32 on the hardware's approximate reciprocal and square root,
+``hip-exact`` float32 with IEEE-rounded division and square root in the reference CPU path's statement order, and
+``hip-f64`` float64."""
 
 
 class BackendUnavailable(RuntimeError):
@@ -64,8 +81,8 @@ class BackendUnavailable(RuntimeError):
 
 
 def select_backend(name: str, tables, num: NumericsRecord, **options):
-    """``auto``, ``gpu`` and ``hip`` take the HIP step; ``hip-exact`` and ``hip-f64`` its other arithmetic. There is no
-    silent CPU fallback: the CPU backend is the reference's own (solver/cpu), so a run without a device fails here."""
+    """The backend a name selects: ``auto``, ``gpu`` and ``hip`` mean the fast float32 HIP step. Without a device this
+    raises; no CPU step stands in for the HIP one."""
     from wadi2d_amd.solver.hip_backend import HipBackend, device_count
 
     key = "hip" if name in ("auto", "gpu") else name
