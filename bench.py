@@ -10,10 +10,12 @@ the device (engine/device_run.py: captured graphs of 64 steps, no host in the lo
 steps from Python through the backend protocol.
 
 Printed: one JSON line. ``value`` counts domain cells (padding excluded) times steps over the wall time of the
-timed region, with the inputs resident in HBM. ``roofline`` is the face kernel's (the dominant one): its
-algorithmic bytes per launch over its mean launch time from HIP events on the library's stream. ``cpu_baseline``
-is the oracle's float instantiation (the reference's float32 statement order, OpenMP over tiles) on the host's
-cores, timed on a bounded number of steps from the same hot state.
+timed region (graph-replayed steps, no events), with the inputs resident in HBM. ``roofline`` is the face kernel's
+(the dominant one): its algorithmic bytes per launch over its mean launch time, from HIP events on the library's
+stream in a pass of its own after the timed steps. ``config.maxima_every_4`` is a further pass with the running
+maxima sampled every 4 steps, as a run samples them by default. ``cpu_baseline`` is the oracle's float
+instantiation (the reference's float32 statement order, OpenMP over tiles) on the host's cores, at least 100 steps
+from the same hot state, with the CPU's model and its nanoseconds per cell-update.
 
 Other workloads (``--workload``): ``ensemble``, member-hours per wall-hour of 64 uneven storm/ARF members run
 together on the device against the same members run one after another the plain way, both held against the
@@ -45,6 +47,10 @@ CELL_BYTES = 52   # h,qx1,qy1,res,src 20 + Hc 4 + level pair 8 + writes qx,qy,h,
 STEP_BYTES = 176  # the whole step as one pass: 140 read + 36 written
 
 
+def domain_cells_of(tables) -> int:
+    return int(tables.grid.mask.sum())
+
+
 def log(*a) -> None:
     print(*a, file=sys.stderr, flush=True)
 
@@ -57,8 +63,14 @@ def main() -> int:
     ap.add_argument("--size", type=int, default=4096)
     ap.add_argument("--hot", type=float, default=5400.0, help="simulated seconds before the timed steps")
     ap.add_argument("--variant", choices=("fast", "exact"), default="fast")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline sample length; 0 skips it")
-    ap.add_argument("--events", choices=("on", "off"), default="on", help="HIP events around the kernels")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="CPU-baseline sample: at least this long")
+    ap.add_argument("--cpu-steps", type=int, default=100, help="CPU-baseline sample: at least this many steps")
+    ap.add_argument("--events", choices=("on", "off"), default="on",
+                    help="a pass with HIP events around the kernels, after the timed steps (device loop); around the "
+                         "timed steps themselves with --loop host")
+    ap.add_argument("--event-steps", type=int, default=200, help="steps of the event-timed pass (device loop)")
+    ap.add_argument("--maxima-steps", type=int, default=500,
+                    help="steps of the pass with the running maxima sampled every 4 steps (device loop); 0 skips it")
     ap.add_argument("--copy-probe", type=float, default=0.0, help="MiB of a plain device copy to time first")
     ap.add_argument("--loop", choices=("device", "host"), default="device")
     ap.add_argument("--graph", type=int, default=64, help="steps per captured graph (device loop)")
@@ -162,18 +174,42 @@ def main() -> int:
                 log(f"[bench] hot start t={state.t:.0f} s steps={state.nsteps} dt={state.dt:.3f} active={b.nact}")
         log(f"[bench] hot start: {time.time() - t0:.1f} s, t={state.t:.0f} s, {state.nsteps} steps")
         run.steps(args.warmup, args.graph)
-        b.timing_read()
-        b.timing(args.events == "on")
+        # the timed steps: captured graphs, no events; wall time only
         n0, tsim0 = run.state[0].nsteps, run.state[0].t
         t0 = time.perf_counter()
         state = run.steps(args.steps, args.graph)[0]  # waits for the last step
         wall = time.perf_counter() - t0
-        b.timing(False)
-        faces_ms, cells_ms, launches = b.timing_read()
         steps, tsim = state.nsteps - n0, state.t - tsim0
         assert not state.done, "the run ended inside the timed steps"
         bal = run.balance(0, b.tally_volume())
-    assert steps == args.steps and launches in (0, steps), (steps, launches)
+        # per-kernel times in a pass of their own: HIP events around each kernel (the loop then runs uncaptured)
+        faces_ms, cells_ms, launches = 0.0, 0.0, 0
+        if args.events == "on" and args.event_steps > 0:
+            b.timing_read()
+            b.timing(True)
+            state = run.steps(args.event_steps, args.graph)[0]
+            b.timing(False)
+            faces_ms, cells_ms, launches = b.timing_read()
+            assert launches == args.event_steps and not state.done, (launches, state.done)
+        # the run loop's own sampling: the running maxima every 4 steps, a run of its own from this state
+        maxima4 = None
+        if args.maxima_steps > 0:
+            b.maxima_set(0.1, 0.05)
+            plan4 = MemberPlan(1, tables.grid.ntiles, rain, bset, state.t, scen.duration_s, state.dt, state.nsteps,
+                               True, rain.state)  # fmt: skip
+            run4 = DeviceRun(b, [plan4], scen.numerics, maxima=4, idle=b.run_idle())
+            run4.steps(args.warmup, args.graph)
+            t1 = time.perf_counter()
+            s4 = run4.steps(args.maxima_steps, args.graph)[0]
+            wall4 = time.perf_counter() - t1
+            assert not s4.done, "the run ended inside the maxima steps"
+            maxima4 = {"maxima_every": 4, "steps": args.maxima_steps, "ms_per_step": wall4 / args.maxima_steps * 1e3,
+                       "value": domain_cells_of(tables) * args.maxima_steps / wall4}
+            log(f"[bench] maxima every 4: {args.maxima_steps} steps in {wall4:.3f} s "
+                f"({maxima4['ms_per_step']:.3f} ms/step)")
+    if args.loop == "host":
+        maxima4 = None
+    assert steps == args.steps and launches in (0, steps, args.event_steps), (steps, launches)
     launches = max(launches, 1)
     domain_cells = int(tables.grid.mask.sum())
     tiles = int(b.nact)
@@ -187,8 +223,9 @@ def main() -> int:
         f"active tiles {tiles}; mass error {bal.relative:.2e}")
 
     cpu = None
-    if args.cpu_seconds > 0:
-        cpu = cpu_baseline(model, tables, scen.numerics, args.cpu_seconds, st.dt if args.loop == "host" else state.dt)
+    if args.cpu_seconds > 0 or args.cpu_steps > 0:
+        cpu = cpu_baseline(model, tables, scen.numerics, args.cpu_seconds, args.cpu_steps,
+                           st.dt if args.loop == "host" else state.dt)
 
     out = {
         "metric": "cell-updates/s",
@@ -216,7 +253,12 @@ def main() -> int:
             "hbm_frac_step": STEP_BYTES * value / HBM_PEAK,
             "faces_ms": face_s * 1e3,
             "cells_ms": cell_s * 1e3,
+            "kernel_times": (f"HIP events, a pass of {launches} steps after the timed ones" if args.loop == "device"
+                             else "HIP events inside the timed steps"),
+            "timed_path": ("captured graphs of %d steps, wall time" % args.graph if args.loop == "device"
+                           else "host loop, wall time"),
             "mass_error": bal.relative,
+            "maxima_every_4": maxima4,
         },
         "roofline": {
             "bound": "hbm",
@@ -443,8 +485,20 @@ def measured_traffic(args, tiles: int, kernel: str):
     return rec["per_launch"][kernel]["bytes"]
 
 
-def cpu_baseline(model, tables, num, seconds: float, dt: float):
-    """The oracle's float32 step on the host's cores, from the GPU's hot state, for about ``seconds``."""
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(model, tables, num, seconds: float, min_steps: int, dt: float):
+    """The oracle's float32 step on the host's cores, from the GPU's hot state: at least ``min_steps`` steps and at
+    least ``seconds`` of stepping."""
     from oracle.oracle import Numerics as ONumerics
     from oracle.oracle import OracleBackend
 
@@ -454,21 +508,28 @@ def cpu_baseline(model, tables, num, seconds: float, dt: float):
     for f in ("h", "d", "u", "v", "mx", "my", "qx", "qy", "res", "om", "src"):
         ob.set_field(f, model.backend.get_field(f))
     ob.set_boundary(model.backend.bc)
-    log(f"[bench] cpu baseline set up in {time.time() - t0:.1f} s on {cores} threads")
-    cells = int(tables.grid.mask.sum())
+    cpu = cpu_model()
+    log(f"[bench] cpu baseline set up in {time.time() - t0:.1f} s on {cores} threads of {cpu}")
+    cells = domain_cells_of(tables)
     done, spent = 0, 0.0
-    while spent < seconds:
+    while spent < seconds or done < min_steps:
         t = time.perf_counter()
         ob.step(dt, done % 8 == 0, done % 16 == 0)
         spent += time.perf_counter() - t
         done += 1
-        log(f"[bench] cpu step {done}: {spent:.1f} s")
+        if done % 10 == 0:
+            log(f"[bench] cpu step {done}: {spent:.1f} s")
+    value = cells * done / spent
     return {
-        "value": cells * done / spent,
+        "value": value,
         "unit": "cell-updates/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"{done} steps of the full {cells}-cell grid from the same hot state (float32, OpenMP over tiles)",
+        "cpu": cpu,
+        "ns_per_cell_step": 1e9 / value,
+        "core_ns_per_cell_step": 1e9 * cores / value,
+        "sample": f"{done} steps ({spent:.1f} s) of the full {cells}-cell grid from the same hot state (float32, "
+                  f"the reference's statement order, OpenMP over tiles on {cores} threads)",
     }
 
 

@@ -713,6 +713,13 @@ struct StepArgs {
     // face momentum: fresh says mx, my hold it (set from the host); else it is the last step's Abar * u, Abar from
     // the mean depths that step read (h_old), recomputed here (a step never stores it)
     int fresh;
+    // device-resident run with the running maxima (every > 0): the cell kernel samples them after the steps whose
+    // count is a multiple of `every`, on every tile without a boundary cell (bnd_tile); those tiles are sampled after
+    // the boundary cells are set (maxima_run_kernel)
+    int every;
+    R arrive, deep;
+    R *mx_d, *mx_h, *mx_s, *mx_z, *mx_t;
+    const int* bnd_tile;
 };
 
 // Both own faces of every own cell of one active tile per workgroup of NTH threads (faces.cu:244-271).
@@ -918,10 +925,13 @@ __device__ __forceinline__ void limit_ring(const R* s_h, const R* s_qx, const R*
 // (cells.cu:28-63), from the ratio and the face discharges before the limit in LDS: writes the limited discharges,
 // the new mean depth, level and residual, the wet share on a wave step, the window's source; returns whether a cell
 // is wet and adds the window's source to src_sum.
+// With `sample`, it then takes the running maxima of the tile's cells at time `time` (the statements of
+// maxima_run_kernel, on the values just computed; `nb` the 3 x 3 block of tiles around t).
 template <class R, bool FAST, int NTH, int ROW0>
 __device__ __forceinline__ bool update_tile(const StepArgs<R>& A, const Cur<R>& s, int t, const R* s_h,
                                             const R* s_qx, const R* s_qy, const R* s_r, R dt, bool want_plan,
-                                            bool new_window, double scale, double& src_sum) {
+                                            bool new_window, double scale, double& src_sum, bool sample = false,
+                                            R time = R(0), const int* nb = nullptr) {
     const R dx = R(A.P.dx);
     const R area = dx * dx;
     const int i = threadIdx.x & (TILE - 1);
@@ -933,6 +943,7 @@ __device__ __forceinline__ bool update_tile(const StepArgs<R>& A, const Cur<R>& 
     R res_v[RT], src_v[RT];
     float hc_v[RT];
     int tag_v[RT];
+    R md_v[RT], mh_v[RT], mt_v[RT];
 #pragma unroll
     for (int r = 0; r < RT; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
@@ -945,8 +956,13 @@ __device__ __forceinline__ bool update_tile(const StepArgs<R>& A, const Cur<R>& 
             src_v[r] = A.src_weight ? (R)(A.src_weight[c] * scale) : R(0);
         else
             src_v[r] = ld_nt(s.src + c);
+        if (sample) {
+            md_v[r] = A.mx_d[c];
+            mh_v[r] = A.mx_h[c];
+            mt_v[r] = A.mx_t[c];
+        }
     }
-    R qw_v[RT], qn_v[RT], hn_v[RT], resn_v[RT], s_v[RT];
+    R qw_v[RT], qn_v[RT], hn_v[RT], resn_v[RT], s_v[RT], dn_v[RT];
     int row_v[RT], mode_v[RT];
     float2 pair_v[RT];
 #pragma unroll
@@ -1020,8 +1036,30 @@ __device__ __forceinline__ bool update_tile(const StepArgs<R>& A, const Cur<R>& 
             A.st.ctag[c] = (unsigned char)row_v[r];
         }
         st_nt(dn, s.d + c);
+        dn_v[r] = dn;
         if (want_plan) st_nt(plan, s.om + c);
         wet |= hn > R(0);
+    }
+    if (sample) {
+        const int te = nb[5], tso = nb[7];  // east and south neighbours
+#pragma unroll
+        for (int r = 0; r < RT; ++r) {
+            const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+            const unsigned c = cs[r];
+            const R dk = dn_v[r], hk = hn_v[r];
+            if (dk > md_v[r]) A.mx_d[c] = dk;
+            if (hk > mh_v[r]) A.mx_h[c] = hk;
+            if (dk > A.deep) {
+                const unsigned ke = i < TILE - 1 ? c + 1 : (unsigned)(te * TILE2 + j * TILE);
+                const unsigned ks = j < TILE - 1 ? c + TILE : (unsigned)(tso * TILE2 + i);
+                const R uc = R(0.5) * (s.u1[c] + s.u1[ke]);
+                const R vc = R(0.5) * (s.v1[c] + s.v1[ks]);
+                const R sp = sqrt(uc * uc + vc * vc);
+                if (sp > A.mx_s[c]) A.mx_s[c] = sp;
+                if (sp * dk > A.mx_z[c]) A.mx_z[c] = sp * dk;
+            }
+            if (mt_v[r] < R(0) && dk >= A.arrive) A.mx_t[c] = time;
+        }
     }
     return wet;
 }
@@ -1054,6 +1092,8 @@ struct TileStep {
     int want, new_window, member, par;
     long long nsteps;
     double scale;
+    int sample;  // the running maxima are sampled after this step, here
+    R time;      // the member's time after the step (the post kernel's t + dt_step)
 };
 template <class R>
 __device__ __forceinline__ bool tile_step(const StepArgs<R>& A, int t, TileStep<R>& ts) {
@@ -1064,6 +1104,8 @@ __device__ __forceinline__ bool tile_step(const StepArgs<R>& A, int t, TileStep<
     ts.member = 0;
     ts.par = A.par;
     ts.nsteps = 0;
+    ts.sample = 0;
+    ts.time = R(0);
     if (A.members) {
         ts.member = uni(A.tile_member[t]);
         const Member& mb = A.members[ts.member];
@@ -1074,6 +1116,10 @@ __device__ __forceinline__ bool tile_step(const StepArgs<R>& A, int t, TileStep<
         ts.scale = mb.scale;
         ts.par = uni(mb.par);
         ts.nsteps = mb.nsteps;
+        if (A.every > 0 && (mb.nsteps + 1) % A.every == 0 && !A.bnd_tile[t]) {
+            ts.sample = 1;
+            ts.time = (R)(mb.t + mb.dt_step);
+        }
     }
     return true;
 }
@@ -1110,7 +1156,7 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     __syncthreads();
     double src_sum = 0.0;
     const bool wet = update_tile<R, FAST, NTH, 2>(A, s, t, s_h, s_qx, s_qy, s_r, ts.dt, ts.want != 0,
-                                                  ts.new_window != 0, ts.scale, src_sum);
+                                                  ts.new_window != 0, ts.scale, src_sum, ts.sample != 0, ts.time, nb);
     tile_flags<NTH>(t, wet, ts.new_window != 0, src_sum, &any_wet, red_src, A.wet, A.src_partial);
 }
 
@@ -1971,16 +2017,18 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     b.par = 1 - b.par;
 }
 
-// Running maxima after the step, at the member's new time (engine/maxima.py:29-51).
+// Running maxima after the step, at the member's new time (engine/maxima.py:29-51), on the listed tiles: those with
+// a boundary cell, whose values the post kernel sets after the cell kernel (the cell kernel samples every other
+// tile).
 template <class R>
-__global__ __launch_bounds__(NT) void maxima_run_kernel(const RunGlobals* __restrict__ g, const Member* __restrict__ mem,
-                                                        const int* __restrict__ tile_member,
-                                                        const int* __restrict__ active, const int* __restrict__ nbr,
+__global__ __launch_bounds__(NT) void maxima_run_kernel(const int* __restrict__ tiles, const int* __restrict__ is_active,
+                                                        const Member* __restrict__ mem,
+                                                        const int* __restrict__ tile_member, const int* __restrict__ nbr,
                                                         R arrive, R deep, Bufs<R> B, R* __restrict__ dmax,
                                                         R* __restrict__ hmax, R* __restrict__ smax,
                                                         R* __restrict__ zmax, R* __restrict__ tarr, int every) {
-    if ((int)blockIdx.x >= g->nact) return;
-    const int t = uni(active[blockIdx.x]);
+    const int t = uni(tiles[blockIdx.x]);
+    if (!is_active[t]) return;
     const Member& b = mem[uni(tile_member[t])];
     // sampled after the steps whose count is a multiple of `every`, as the host loop's on_step does; the member's
     // parities have moved on, so its current buffers hold the step's results
@@ -2093,6 +2141,8 @@ struct w2d_handle {
     RunNum run_num{};
     int *run_tile_member = nullptr, *run_has_source = nullptr, *run_bnd_tile = nullptr, *run_is_active = nullptr,
         *run_idle = nullptr, *run_flags = nullptr, *run_woken = nullptr, *run_leaving = nullptr, *run_seg = nullptr;
+    int* run_bnd_list = nullptr;  // the tiles with a boundary cell (run_nbnd of them), for maxima_run_kernel
+    int run_nbnd = 0;
     double *run_scales = nullptr, *run_src_partial = nullptr;
     unsigned long long* run_wave = nullptr;  // per member, the step's largest wave speed as bits
     int run_maxima = 0;
@@ -3382,6 +3432,17 @@ int enqueue_run_step(w2d_handle* H) {
     A.src_partial = H->run_src_partial;
     A.member_wave = H->run_wave;
     A.excess = H->excess;
+    if (H->run_maxima) {
+        A.every = H->run_maxima;
+        A.arrive = (R)H->m_arrive;
+        A.deep = (R)H->m_deep;
+        A.mx_d = (R*)H->maxima[0];
+        A.mx_h = (R*)H->maxima[1];
+        A.mx_s = (R*)H->maxima[2];
+        A.mx_z = (R*)H->maxima[3];
+        A.mx_t = (R*)H->maxima[4];
+        A.bnd_tile = H->run_bnd_tile;
+    }
     const int nt = H->ntiles;
     int s;
     if (H->excess) {
@@ -3396,10 +3457,11 @@ int enqueue_run_step(w2d_handle* H) {
                                                      H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
                                                      H->b_d0, H->b_h0, A.b, H->t_face, H->t_axis, H->t_sign,
                                                      H->t_total, H->t_flow);
-    if (H->run_maxima)
-        maxima_run_kernel<R><<<nt, NT, 0, H->stream>>>(
-            H->run_glob, H->run_mem, H->run_tile_member, H->active, H->nbr, (R)H->m_arrive, (R)H->m_deep, A.b,
-            (R*)H->maxima[0], (R*)H->maxima[1], (R*)H->maxima[2], (R*)H->maxima[3], (R*)H->maxima[4], H->run_maxima);
+    if (H->run_maxima && H->run_nbnd)
+        maxima_run_kernel<R><<<H->run_nbnd, NT, 0, H->stream>>>(
+            H->run_bnd_list, H->run_is_active, H->run_mem, H->run_tile_member, H->nbr, (R)H->m_arrive,
+            (R)H->m_deep, A.b, (R*)H->maxima[0], (R*)H->maxima[1], (R*)H->maxima[2], (R*)H->maxima[3],
+            (R*)H->maxima[4], H->run_maxima);
     HIP_TRY(hipGetLastError());
     H->m_fresh = 0;
     H->run_dirty = 1;
@@ -3582,6 +3644,17 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
         (s = up(&H->run_idle, idle_v)) || (s = up(&H->run_flags, zeros)) || (s = up(&H->run_woken, zeros)) ||
         (s = up(&H->run_leaving, zeros)) || (s = up(&H->run_seg, seg)))
         return s;
+    std::vector<int32_t> bnd_list;
+    for (int t = 0; t < nt; ++t)
+        if (bnd_tile[t]) bnd_list.push_back(t);
+    if (H->run_bnd_list) {
+        H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_bnd_list), H->owned.end());
+        (void)hipFree(H->run_bnd_list);
+        H->run_bnd_list = nullptr;
+    }
+    H->run_nbnd = (int)bnd_list.size();
+    if (bnd_list.empty()) bnd_list.push_back(0);
+    if ((s = upload(H, &H->run_bnd_list, bnd_list.data(), bnd_list.size()))) return s;
     if (H->run_mem) {
         H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_mem), H->owned.end());
         (void)hipFree(H->run_mem);
