@@ -221,6 +221,44 @@ int w2d_closure_set(w2d_handle* h, int32_t n, const int32_t* cell_r, const int32
 int w2d_set_launch(w2d_handle* h, int32_t faces_threads, int32_t cells_threads);
 /* synchronize (gpu/backend.py:202-203). */
 int w2d_synchronize(w2d_handle* h);
+
+/* ---- Local time stepping: the operations of the level stepper, one call per method of GpuLevels
+ * (solver/gpu/levels.py:11-203, binding solver/gpu/kernels/levels.cu:1-172 and the tile-list faces, limit and
+ * update). A level run keeps the handle's buffer parity: the depth is updated in place, the face velocities and
+ * momenta step into the other buffers and are committed tile by tile. Tiles hold no halos, so the reference's
+ * halo exchanges (faces_done, limit_done) have no call; halo_state only sets the smallest step at which the staged
+ * neighbours are read, between a coarser neighbour's old and new depth and level.
+ * setup: allocates the ratio, old depth/level and edge estimates, and stores the face momentum. */
+int w2d_level_setup(w2d_handle* h);
+/* set_level (gpu/levels.py:61-62): every tile's time level. */
+int w2d_level_set(w2d_handle* h, const int32_t* level);
+/* lists (gpu/levels.py:58-59): a list of tiles (width 1) or of edge rows (width 4: fine tile, side of the fine tile
+ * 0 W 1 N 2 E 3 S, coarse tile, edge number < 4 ntiles), copied to the device; `id` names it until lists_clear. */
+int w2d_level_list(w2d_handle* h, const int32_t* a, int32_t n, int32_t width, int32_t* id);
+int w2d_level_lists_clear(w2d_handle* h);
+/* halo_state (gpu/levels.py:64-80, halo_levels levels.cu:67-91): the smallest step `sub` of the cycle. */
+int w2d_level_halo_state(w2d_handle* h, int32_t sub);
+/* faces (gpu/levels.py:96-103; faces.cu:244-271 over a list at one dt). */
+int w2d_level_faces(w2d_handle* h, int32_t tiles, double dt, int32_t want_wave);
+/* strips (gpu/levels.py:105-106; levels.cu:34-64): edge faces the coarse tile owns, at the fine rate. */
+int w2d_level_strips(w2d_handle* h, int32_t rows, double dt, int32_t want_wave);
+/* limit (gpu/levels.py:146-150; cells.cu:4-18 over a list at one dt). */
+int w2d_level_limit(w2d_handle* h, int32_t tiles, double dt);
+/* save (gpu/levels.py:155-157; save_old levels.cu:93-105). */
+int w2d_level_save(w2d_handle* h, int32_t tiles);
+/* estimate (gpu/levels.py:159-161; edge_estimate levels.cu:143-154). */
+int w2d_level_estimate(w2d_handle* h, int32_t rows);
+/* flux (gpu/levels.py:163-168; edge_flux levels.cu:156-172): one axis's rows per call. */
+int w2d_level_flux(w2d_handle* h, int32_t rows, double dt);
+/* update (gpu/levels.py:175-181; cells.cu:28-63 with debt): every listed tile's wet flag is written. */
+int w2d_level_update(w2d_handle* h, int32_t tiles, double dt, int32_t want_plan);
+/* commit and commit_strips (gpu/levels.py:189-199; levels.cu:107-140). */
+int w2d_level_commit(w2d_handle* h, int32_t tiles);
+int w2d_level_commit_strips(w2d_handle* h, int32_t rows);
+/* clear_wave (what 0) and clear_wet (what 1) (gpu/levels.py:86-90). */
+int w2d_level_clear(w2d_handle* h, int32_t what);
+/* boundary (gpu/levels.py:183-187; cells.cu boundary) on the in-place depth. */
+int w2d_level_boundary(w2d_handle* h);
 const char* w2d_last_error(void);
 
 /* ---- The learned face closure's network (16 -> 64 -> 64 -> 2, ReLU; BF16 weights on matrix cores, FP32

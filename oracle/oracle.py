@@ -7,6 +7,8 @@ What it restates, file by file:
   kernels and step order ...... oracle/step_oracle.cpp (wadi2d/solver/cpu/kernels.py, cpu/backend.py:95-126)
   CpuBackend protocol ......... OracleBackend   (wadi2d/solver/cpu/backend.py:14-141)
   Stepper ..................... OracleStepper   (wadi2d/solver/stepper.py:14-129)
+  local time stepping ......... oracle/levels.py (wadi2d/solver/level_stepper.py, solver/levels.py,
+                                solver/cpu/levels.py)
   Model ....................... OracleModel     (wadi2d/solver/model.py:32-185)
   host state conversions ...... to_state/from_state/level_of/plan_of (wadi2d/solver/state.py:198-301)
   water ledger ................ Ledger          (wadi2d/engine/monitor.py:35-79,197-248; engine/maxima.py:28-192)
@@ -80,6 +82,10 @@ def lib() -> ctypes.CDLL:
         L.oracle_refresh_plan.argtypes = [VP, VP, I32]
         L.oracle_refresh_halos.argtypes = [VP]
         L.oracle_set_threads.argtypes = [VP, I32]
+        L.oracle_lts_init.argtypes = [VP]
+        L.oracle_lts_set_level.argtypes = [VP, VP]
+        L.oracle_lts_tiles.argtypes = [VP, I32, VP, I32, D, I32, I32]
+        L.oracle_lts_edges.argtypes = [VP, I32, VP, I32, D, I32]
         _lib = L
     return _lib
 
@@ -102,6 +108,10 @@ class Numerics:
     wave_floor: float = 3.0
     plan_min: float = 0.02
     active_every: int = 16
+    local_time_stepping: bool = False
+    lts_levels: int = 3
+    lts_wave_floor: float = 1.0
+    lts_buffer: float = 1.0
 
     @classmethod
     def like(cls, other) -> Numerics:
@@ -197,7 +207,7 @@ class OracleBackend:
         t = _Tables(self.grid.ntiles, tables.rows, nbr.ctypes.data,
                     *[f32(getattr(tables, n)) for n in ("hc", "dtab", "flx", "frx", "glx", "grx", "efx", "cwx",
                       "ftabx", "fly", "fry", "gly", "gry", "efy", "cwy", "ftaby", "zcx", "zcy")])  # fmt: skip
-        lead = float(num.dt_every)  # backend.py:84 without local time stepping
+        lead = float(max(num.dt_every, 1 << num.lts_levels) if num.local_time_stepping else num.dt_every)  # backend.py:32
         p = _Params(G, self.grid.dx, num.e_wet, num.h_cfl, num.u_max, lead, num.plan_min, tables.rows,
                     int(num.limiter != "none"), int(num.sheet_flow), int(num.mirror_dry_faces),
                     int(num.face_depth_carry))  # fmt: skip

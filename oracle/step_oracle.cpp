@@ -23,6 +23,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 namespace {
@@ -272,6 +273,7 @@ enum Field {
 
 template <class R>
 struct Model {
+    using Real = R;
     Params P;
     int ntiles = 0;
     int nhalo = 0;
@@ -312,17 +314,19 @@ struct Model {
             }
     }
 
-    void exchange(R* arr) {  // kernels.py:314-319; a pure copy from own entries into halo entries
-        const int n = (int)active.size();
+    void exchange(R* arr) { exchange_list(active.data(), (int)active.size(), arr); }
+
+    void exchange_list(const int32_t* tiles, int n, R* arr) {  // kernels.py:314-319; own entries into halo entries
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int a = 0; a < n; ++a) {
-            const int t = active[a];
+            const int t = tiles[a];
             for (int m = 0; m < nhalo; ++m) arr[(size_t)t * STRIDE2 + slot[m]] = arr[src[(size_t)(t - 1) * nhalo + m]];
         }
     }
 
-    void faces(R dt, bool want_wave) {  // kernels.py:222-244
-        const int n = (int)active.size();
+    void faces(R dt, bool want_wave) { faces_list(active.data(), (int)active.size(), dt, want_wave); }
+
+    void faces_list(const int32_t* tiles, int n, R dt, bool want_wave) {  // kernels.py:222-244
         const FaceStatics<R> sx{flx.data(), frx.data(), glx.data(), grx.data(), efx.data(), cwx.data(), ftabx.data()};
         const FaceStatics<R> sy{fly.data(), fry.data(), gly.data(), gry.data(), efy.data(), cwy.data(), ftaby.data()};
         R *h = a[F_H].data(), *d = a[F_D].data(), *u = a[F_U].data(), *v = a[F_V].data();
@@ -331,7 +335,7 @@ struct Model {
         R *qx1 = a[F_QX1].data(), *qy1 = a[F_QY1].data(), *om = a[F_OM].data();
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 1)
         for (int ai = 0; ai < n; ++ai) {
-            const int t = active[ai];
+            const int t = tiles[ai];
             R w = R(0);
             for (int j = 0; j < TILE; ++j)
                 for (int i = 0; i < TILE; ++i) {
@@ -347,15 +351,16 @@ struct Model {
         }
     }
 
-    void limit(R dt) {  // kernels.py:247-262
+    void limit(R dt) { limit_list(active.data(), (int)active.size(), dt); }
+
+    void limit_list(const int32_t* tiles, int n, R dt) {  // kernels.py:247-262
         const R dx = R(P.dx);
         const R area = dx * dx;
-        const int n = (int)active.size();
         const R *h = a[F_H].data(), *qx1 = a[F_QX1].data(), *qy1 = a[F_QY1].data();
         R* r = a[F_R].data();
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int ai = 0; ai < n; ++ai) {
-            const int t = active[ai];
+            const int t = tiles[ai];
             for (int j = 0; j < TILE; ++j)
                 for (int i = 0; i < TILE; ++i) {
                     const int k = t * STRIDE2 + (j + HALO) * STRIDE + i + HALO;
@@ -367,17 +372,21 @@ struct Model {
         }
     }
 
-    void update(R dt, bool want_plan) {  // kernels.py:270-299, debt off (cpu/backend.py:432-435)
+    void update(R dt, bool want_plan) {  // debt off in the uniform step (cpu/backend.py:432-435)
+        update_list(active.data(), (int)active.size(), dt, want_plan, false);
+    }
+
+    // kernels.py:270-299; `debt` keeps an overdrawn cell's shortfall in its residual (the level stepper's form)
+    void update_list(const int32_t* tiles, int n, R dt, bool want_plan, bool debt) {
         const R dx = R(P.dx);
         const R area = dx * dx;
         const int rows = P.rows;
-        const int n = (int)active.size();
         R *res = a[F_RES].data(), *om = a[F_OM].data(), *qx1 = a[F_QX1].data(), *qy1 = a[F_QY1].data();
         R *r = a[F_R].data(), *srcv = a[F_SRC].data(), *h = a[F_H].data(), *d = a[F_D].data();
         R *qx = a[F_QX].data(), *qy = a[F_QY].data();
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (int ai = 0; ai < n; ++ai) {
-            const int t = active[ai];
+            const int t = tiles[ai];
             int any_wet = 0;
             for (int j = 0; j < TILE; ++j)
                 for (int i = 0; i < TILE; ++i) {
@@ -394,7 +403,7 @@ struct Model {
                     const R inc = res[k] + dt * srcv[k] - dt * (qe - qw + qs - qn) / area;
                     const R total = h0 + inc;
                     const R hn = std::max(total, R(0));
-                    res[k] = total > R(0) ? inc - (total - h0) : R(0);
+                    res[k] = total > R(0) ? inc - (total - h0) : (debt ? total : R(0));
                     h[k] = hn;
                     const float* tab = dtab.data() + (size_t)rows * fid;
                     d[k] = cell_level(hn, R(hc[fid]), tab, rows);
@@ -459,6 +468,163 @@ struct Model {
         std::swap(a[F_V], a[F_V1]);
         std::swap(a[F_MX], a[F_MX1]);
         std::swap(a[F_MY], a[F_MY1]);
+    }
+
+    // ---- local time stepping: the level stepper's operations (wadi2d/solver/cpu/levels.py:1-147), over tile lists
+    // and edge rows (fine tile, side of the fine tile 0 W 1 N 2 E 3 S, coarse tile, edge number)
+    std::vector<R> h0, d0, est;
+    std::vector<int32_t> level;
+
+    void lts_init() {
+        if (!h0.empty()) return;
+        h0.assign((size_t)ntiles * STRIDE2, R(0));
+        d0.assign((size_t)ntiles * STRIDE2, R(0));
+        est.assign((size_t)4 * ntiles * TILE, R(0));
+        level.assign(ntiles, 0);
+    }
+
+    static int edge_face(int t, int side, int q) {  // cpu/levels.py:12-21
+        const int base = t * STRIDE2;
+        if (side == 0) return base + (q + HALO) * STRIDE + HALO;
+        if (side == 1) return base + HALO * STRIDE + q + HALO;
+        if (side == 2) return base + (q + HALO) * STRIDE + HALO + TILE;
+        return base + (HALO + TILE) * STRIDE + q + HALO;
+    }
+
+    static int edge_cell(int c, int side, int q) {  // cpu/levels.py:24-33
+        const int base = c * STRIDE2;
+        if (side == 0) return base + (q + HALO) * STRIDE + HALO + TILE - 1;
+        if (side == 1) return base + (HALO + TILE - 1) * STRIDE + q + HALO;
+        if (side == 2) return base + (q + HALO) * STRIDE + HALO;
+        return base + HALO * STRIDE + q + HALO;
+    }
+
+    void strips(const int32_t* e, int ne, R dt, bool want_wave) {  // cpu/levels.py:36-57
+        const FaceStatics<R> sx{flx.data(), frx.data(), glx.data(), grx.data(), efx.data(), cwx.data(), ftabx.data()};
+        const FaceStatics<R> sy{fly.data(), fry.data(), gly.data(), gry.data(), efy.data(), cwy.data(), ftaby.data()};
+        R *h = a[F_H].data(), *d = a[F_D].data(), *u = a[F_U].data(), *v = a[F_V].data();
+        R *mx = a[F_MX].data(), *my = a[F_MY].data(), *qx = a[F_QX].data(), *qy = a[F_QY].data();
+        R *u1 = a[F_U1].data(), *v1 = a[F_V1].data(), *mx1 = a[F_MX1].data(), *my1 = a[F_MY1].data();
+        R *qx1 = a[F_QX1].data(), *qy1 = a[F_QY1].data(), *om = a[F_OM].data();
+        for (int n = 0; n < ne; ++n) {
+            const int fine = e[4 * n], side = e[4 * n + 1], t = e[4 * n + 2];
+            for (int q = 0; q < TILE; ++q) {
+                R w;
+                if (side == 2) {
+                    const int k = t * STRIDE2 + (q + HALO) * STRIDE + HALO;
+                    w = face_step<R>(k, t * TILE2 + q * TILE, dt, want_wave, om, zcx.data(), 1, STRIDE, h, d, u, mx, qx,
+                                     qy, v, sx, u1, mx1, qx1, P);
+                } else {
+                    const int k = t * STRIDE2 + HALO * STRIDE + q + HALO;
+                    w = face_step<R>(k, t * TILE2 + q, dt, want_wave, om, zcy.data(), STRIDE, 1, h, d, v, my, qy, qx,
+                                     u, sy, v1, my1, qy1, P);
+                }
+                if (want_wave) {
+                    wave[t] = std::max(wave[t], w);
+                    wave[fine] = std::max(wave[fine], w);
+                }
+            }
+        }
+    }
+
+    // cpu/levels.py:60-80: the halos of the listed tiles, a source tile in the middle of its step read at time n
+    // between its old and new values
+    void halo_levels(const int32_t* tiles, int nt, int n) {
+        R *h = a[F_H].data(), *d = a[F_D].data(), *u = a[F_U].data(), *v = a[F_V].data();
+        R *qx = a[F_QX].data(), *qy = a[F_QY].data();
+        for (int x = 0; x < nt; ++x) {
+            const int t = tiles[x];
+            for (int m = 0; m < nhalo; ++m) {
+                const int dst = t * STRIDE2 + slot[m];
+                const int s = src[(size_t)(t - 1) * nhalo + m];
+                const int span = 1 << level[s / STRIDE2];
+                const int part = n & (span - 1);
+                if (part == 0) {
+                    h[dst] = h[s];
+                    d[dst] = d[s];
+                } else {
+                    const R w = R(part) / R(span);
+                    h[dst] = h0[s] + w * (h[s] - h0[s]);
+                    d[dst] = d0[s] + w * (d[s] - d0[s]);
+                }
+                u[dst] = u[s];
+                v[dst] = v[s];
+                qx[dst] = qx[s];
+                qy[dst] = qy[s];
+            }
+        }
+    }
+
+    void save_old(const int32_t* tiles, int nt) {  // cpu/levels.py:83-90
+        const R *h = a[F_H].data(), *d = a[F_D].data();
+        for (int x = 0; x < nt; ++x)
+            for (int j = 0; j < TILE; ++j)
+                for (int i = 0; i < TILE; ++i) {
+                    const int k = tiles[x] * STRIDE2 + (j + HALO) * STRIDE + i + HALO;
+                    h0[k] = h[k];
+                    d0[k] = d[k];
+                }
+    }
+
+    void commit(const int32_t* tiles, int nt) {  // cpu/levels.py:93-102
+        for (int x = 0; x < nt; ++x)
+            for (int j = 0; j < TILE; ++j)
+                for (int i = 0; i < TILE; ++i) {
+                    const int k = tiles[x] * STRIDE2 + (j + HALO) * STRIDE + i + HALO;
+                    a[F_U][k] = a[F_U1][k];
+                    a[F_V][k] = a[F_V1][k];
+                    a[F_MX][k] = a[F_MX1][k];
+                    a[F_MY][k] = a[F_MY1][k];
+                }
+    }
+
+    void commit_strips(const int32_t* e, int ne) {  // cpu/levels.py:105-116
+        for (int n = 0; n < ne; ++n) {
+            const int t = e[4 * n + 2];
+            for (int q = 0; q < TILE; ++q) {
+                if (e[4 * n + 1] == 2) {
+                    const int k = t * STRIDE2 + (q + HALO) * STRIDE + HALO;
+                    a[F_U][k] = a[F_U1][k];
+                    a[F_MX][k] = a[F_MX1][k];
+                } else {
+                    const int k = t * STRIDE2 + HALO * STRIDE + q + HALO;
+                    a[F_V][k] = a[F_V1][k];
+                    a[F_MY][k] = a[F_MY1][k];
+                }
+            }
+        }
+    }
+
+    void edge_estimate(const int32_t* e, int ne) {  // cpu/levels.py:119-127
+        const R *qx1 = a[F_QX1].data(), *qy1 = a[F_QY1].data(), *r = a[F_R].data();
+        for (int n = 0; n < ne; ++n) {
+            const int side = e[4 * n + 1];
+            for (int q = 0; q < TILE; ++q) {
+                const int k = edge_face(e[4 * n], side, q);
+                const int s = (side & 1) ? STRIDE : 1;
+                const R f = (side & 1) ? qy1[k] : qx1[k];
+                est[(size_t)e[4 * n + 3] * TILE + q] = scaled(f, r[k - s], r[k]);
+            }
+        }
+    }
+
+    void edge_flux(const int32_t* e, int ne, R dt) {  // cpu/levels.py:130-147
+        const R dx = R(P.dx);
+        const R *qx1 = a[F_QX1].data(), *qy1 = a[F_QY1].data(), *r = a[F_R].data();
+        R *res = a[F_RES].data(), *qx = a[F_QX].data(), *qy = a[F_QY].data();
+        for (int n = 0; n < ne; ++n) {
+            const int side = e[4 * n + 1], c = e[4 * n + 2];
+            for (int q = 0; q < TILE; ++q) {
+                const int k = edge_face(e[4 * n], side, q);
+                const int s = (side & 1) ? STRIDE : 1;
+                const R f = (side & 1) ? qy1[k] : qx1[k];
+                const R qs = scaled(f, r[k - s], r[k]);
+                const R sign = side < 2 ? R(-1) : R(1);
+                res[edge_cell(c, side, q)] += sign * dt * (qs - est[(size_t)e[4 * n + 3] * TILE + q]) / (dx * dx);
+                if (side == 2) qx[c * STRIDE2 + (q + HALO) * STRIDE + HALO] = qs;
+                if (side == 3) qy[c * STRIDE2 + HALO * STRIDE + q + HALO] = qs;
+            }
+        }
     }
 
     void refresh_halos() {  // cpu/backend.py:411-414
@@ -646,6 +812,69 @@ void oracle_set_wet(OracleHandle* h, const int32_t* flags) {
 void oracle_refresh_plan(OracleHandle* h, const int32_t* tiles, int n) { DISPATCH(h, plan(tiles, n)); }
 
 void oracle_refresh_halos(OracleHandle* h) { DISPATCH(h, refresh_halos()); }
+
+// ---- local time stepping (cpu/levels.py CpuLevels): one call per operation of the level stepper
+void oracle_lts_init(OracleHandle* h) { DISPATCH(h, lts_init()); }
+
+void oracle_lts_set_level(OracleHandle* h, const int32_t* level) {
+    auto set = [&](auto* m) { std::copy(level, level + m->ntiles, m->level.begin()); };
+    if (h->prec == 8)
+        set(M<double>(h));
+    else
+        set(M<float>(h));
+}
+
+// op: 0 halo_state (arg n, flag want_plan), 1 faces, 2 faces_done, 3 limit, 4 limit_done, 5 save, 6 update (flag
+// want_plan), 7 commit, 8 clear_wave, 9 clear_wet, 10 boundary, 11 refresh_plan
+void oracle_lts_tiles(OracleHandle* h, int op, const int32_t* tiles, int n, double dt, int flag, int arg) {
+    auto run = [&](auto* m) {
+        using R = typename std::remove_pointer<decltype(m)>::type::Real;
+        const R d = R(dt);
+        switch (op) {
+        case 0:
+            m->halo_levels(tiles, n, arg);
+            if (flag) m->exchange_list(tiles, n, m->a[F_OM].data());
+            break;
+        case 1: m->faces_list(tiles, n, d, flag != 0); break;
+        case 2:
+            for (Field f : {F_U1, F_V1, F_QX1, F_QY1}) m->exchange_list(tiles, n, m->a[f].data());
+            break;
+        case 3: m->limit_list(tiles, n, d); break;
+        case 4: m->exchange_list(tiles, n, m->a[F_R].data()); break;
+        case 5: m->save_old(tiles, n); break;
+        case 6: m->update_list(tiles, n, d, flag != 0, true); break;
+        case 7: m->commit(tiles, n); break;
+        case 8: std::fill(m->wave.begin(), m->wave.end(), R(0)); break;
+        case 9: std::fill(m->wet.begin(), m->wet.end(), 0); break;
+        case 10:
+            if (!m->bcell.empty()) m->boundary();
+            break;
+        case 11: m->plan(tiles, n); break;
+        }
+    };
+    if (h->prec == 8)
+        run(M<double>(h));
+    else
+        run(M<float>(h));
+}
+
+// op: 0 strips (flag want_wave), 1 commit_strips, 2 estimate, 3 flux; `edges` rows of 4
+void oracle_lts_edges(OracleHandle* h, int op, const int32_t* edges, int n, double dt, int flag) {
+    auto run = [&](auto* m) {
+        using R = typename std::remove_pointer<decltype(m)>::type::Real;
+        const R d = R(dt);
+        switch (op) {
+        case 0: m->strips(edges, n, d, flag != 0); break;
+        case 1: m->commit_strips(edges, n); break;
+        case 2: m->edge_estimate(edges, n); break;
+        case 3: m->edge_flux(edges, n, d); break;
+        }
+    };
+    if (h->prec == 8)
+        run(M<double>(h));
+    else
+        run(M<float>(h));
+}
 
 void oracle_set_threads(OracleHandle* h, int threads) {
     if (h->prec == 8)

@@ -36,3 +36,17 @@ def sloped_valley(nx: int = 220, ny: int = 70, k: int = 4, dx: float = 10.0, see
     z[wall & ~gap] += 1.5
     tables = build_tables(tg, ArrayBed(z, pixel), Tables(rows=rows), 0.04)
     return tables
+
+
+def deep_and_shallow(n: int = 200, k: int = 4, dx: float = 10.0, seed: int = 11, deep: float = 4.0,
+                     shallow: float = 0.05):  # fmt: skip
+    """A rough plain, deep water (``deep`` m) on its west fifth and a thin sheet (``shallow`` m) everywhere else: wave
+    speeds from the floor to about 7 m/s across the grid, so the level stepper holds tiles at several time levels.
+    Returns (tables, start depth)."""
+    rng = np.random.default_rng(seed)
+    tg = TileGrid.from_mask(np.ones((n, n), bool), 0.0, 0.0, dx)
+    z = gaussian_filter(rng.normal(0, 1.0, (tg.shape[0] * k + 2, tg.shape[1] * k + 2)), 4) * 2.0 + 100.0
+    tables = build_tables(tg, ArrayBed(z, dx / k), Tables(), 0.035)
+    h0 = np.full(tg.shape, shallow)
+    h0[:, : n // 5] = deep
+    return tables, h0

@@ -2071,6 +2071,8 @@ __global__ void fill_kernel(R* a, size_t n, R v) {
     for (size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x; p < n; p += (size_t)gridDim.x * blockDim.x) a[p] = v;
 }
 
+#include "levels_kernels.hpp"
+
 }  // namespace
 
 // ------------------------------------------------------------------------------------------------------------
@@ -2146,6 +2148,13 @@ struct w2d_handle {
     double *run_scales = nullptr, *run_src_partial = nullptr;
     unsigned long long* run_wave = nullptr;  // per member, the step's largest wave speed as bits
     int run_maxima = 0;
+    // local time stepping (w2d_level_*): ratio, old depth and level, edge estimates, per-tile levels; the lists of
+    // the current plan; the smallest step the halos are read at
+    void *lv_r = nullptr, *lv_h0 = nullptr, *lv_d0 = nullptr, *lv_est = nullptr;
+    int* lv_level = nullptr;
+    std::vector<int*> lv_lists;
+    std::vector<int> lv_count, lv_width;
+    int lv_sub = 0;
     void* cells_buf = nullptr;  // w2d_read_cells: the indices and values of a gather, grown as needed
     size_t cells_cap = 0;
     // steps were enqueued since the members' parities were last brought back to the handle's (run_align), or the
@@ -3822,3 +3831,5 @@ int w2d_run_idle(w2d_handle* H, int32_t* out) {
 }
 
 }  // extern "C"
+
+#include "levels_host.hpp"

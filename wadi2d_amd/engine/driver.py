@@ -81,6 +81,11 @@ class Driver:
         self.last_t = model.stepper.t
         self.dry = False
         self.budget: int | None = None
+        # the level stepper calls back on every smallest step, where the boundary faces carry their flow; the tally
+        # counts there (monitor.py:111-113), not once per cycle
+        self.counted_inside = hasattr(model.stepper, "substep_hook")
+        if self.counted_inside:
+            model.stepper.substep_hook = self.b.tally_add
 
     def storage(self) -> float:
         return self.b.storage_sum() * self.area
@@ -93,7 +98,8 @@ class Driver:
 
     def on_step(self, st) -> None:
         """After every step: the boundary tally, the maxima every ``maxima_every`` steps, the step budget."""
-        self.b.tally_add(st.t - self.last_t)
+        if not self.counted_inside:
+            self.b.tally_add(st.t - self.last_t)
         self.last_t = st.t
         if self.peak_flow:
             self.peak_q = max(self.peak_q, self.b.tally_flow())

@@ -368,6 +368,8 @@ class _DeviceLoop:
 
         if not hasattr(model.backend, "run_setup"):
             raise InvalidJob("the device loop needs a HIP backend")
+        if cfg.numerics.local_time_stepping:
+            raise InvalidJob("the device loop steps uniformly: local time stepping runs with the host loop")
         self.cfg, self.model, self.drv, self.monitor = cfg, model, drv, monitor
         st = model.stepper
         plan = MemberPlan(1, model.grid.ntiles, rain, bset, st.t, end, st.dt, st.nsteps, st.refresh_due, rain.state,

@@ -116,6 +116,22 @@ _SIGS = {
     "w2d_tally_totals": [VP, VP],
     "w2d_copy_probe": [I64, I32, ctypes.POINTER(D)],
     "w2d_timing_read": [VP, ctypes.POINTER(D), ctypes.POINTER(D), ctypes.POINTER(I64)],
+    "w2d_level_setup": [VP],
+    "w2d_level_set": [VP, VP],
+    "w2d_level_list": [VP, VP, I32, I32, ctypes.POINTER(I32)],
+    "w2d_level_lists_clear": [VP],
+    "w2d_level_halo_state": [VP, I32],
+    "w2d_level_faces": [VP, I32, D, I32],
+    "w2d_level_strips": [VP, I32, D, I32],
+    "w2d_level_limit": [VP, I32, D],
+    "w2d_level_save": [VP, I32],
+    "w2d_level_estimate": [VP, I32],
+    "w2d_level_flux": [VP, I32, D],
+    "w2d_level_update": [VP, I32, D, I32],
+    "w2d_level_commit": [VP, I32],
+    "w2d_level_commit_strips": [VP, I32],
+    "w2d_level_clear": [VP, I32],
+    "w2d_level_boundary": [VP],
 }
 
 
@@ -290,6 +306,10 @@ class HipBackend:
         n = int(arrs[0].size)
         check(api().w2d_closure_set(self.h, n, *[a.ctypes.data for a in arrs], int(conveyance)))
 
+    def level_ops(self) -> HipLevels:
+        """The level stepper's operations on this backend (gpu/backend.py:190-191)."""
+        return HipLevels(self)
+
     def set_launch(self, faces_threads: int = 512, cells_threads: int = 256) -> None:
         """Workgroup sizes of the face and cell kernels (256 or 512 each); the fields do not depend on them."""
         check(api().w2d_set_launch(self.h, int(faces_threads), int(cells_threads)))
@@ -445,3 +465,78 @@ class HipBackend:
         out = D(0.0)
         check(api().w2d_storage(self.h, ctypes.byref(out)))
         return float(out.value)
+
+
+class HipLevels:
+    """The level stepper's operations on the device (the methods of GpuLevels, wadi2d/solver/gpu/levels.py:11-203)
+    over the ``w2d_level_*`` calls. ``lists`` turns a host array into a device list handle; the tiles carry no halos,
+    so ``faces_done`` and ``limit_done`` do nothing, and ``halo_state`` only sets the smallest step the staged
+    neighbours are read at."""
+
+    def __init__(self, backend: HipBackend) -> None:
+        self.b = backend
+        self.h = backend.h
+        check(api().w2d_level_setup(self.h))
+
+    def lists(self, a: np.ndarray) -> int:
+        arr = np.ascontiguousarray(a, dtype=np.int32)
+        width = 4 if arr.ndim == 2 else 1
+        out = I32(0)
+        n = arr.shape[0] if arr.ndim == 2 else arr.size
+        check(api().w2d_level_list(self.h, arr.ctypes.data, int(n), width, ctypes.byref(out)))
+        return int(out.value)
+
+    def clear_lists(self) -> None:
+        check(api().w2d_level_lists_clear(self.h))
+
+    def set_level(self, level: np.ndarray) -> None:
+        lv = np.ascontiguousarray(level, dtype=np.int32)
+        check(api().w2d_level_set(self.h, lv.ctypes.data))
+
+    def halo_state(self, tiles: int, n: int, want_plan: bool) -> None:
+        check(api().w2d_level_halo_state(self.h, int(n)))
+
+    def clear_wave(self) -> None:
+        check(api().w2d_level_clear(self.h, 0))
+
+    def clear_wet(self) -> None:
+        check(api().w2d_level_clear(self.h, 1))
+
+    def faces(self, tiles: int, dt: float, want_wave: bool) -> None:
+        check(api().w2d_level_faces(self.h, tiles, float(dt), int(want_wave)))
+
+    def strips(self, rows: int, dt: float, want_wave: bool) -> None:
+        check(api().w2d_level_strips(self.h, rows, float(dt), int(want_wave)))
+
+    def faces_done(self, tiles: int) -> None:
+        return None
+
+    def limit(self, tiles: int, dt: float) -> None:
+        check(api().w2d_level_limit(self.h, tiles, float(dt)))
+
+    def limit_done(self, tiles: int) -> None:
+        return None
+
+    def save(self, tiles: int) -> None:
+        check(api().w2d_level_save(self.h, tiles))
+
+    def estimate(self, rows: int) -> None:
+        check(api().w2d_level_estimate(self.h, rows))
+
+    def flux(self, rows: int, dt: float) -> None:
+        check(api().w2d_level_flux(self.h, rows, float(dt)))
+
+    def update(self, tiles: int, dt: float, want_plan: bool) -> None:
+        check(api().w2d_level_update(self.h, tiles, float(dt), int(want_plan)))
+
+    def boundary(self) -> None:
+        check(api().w2d_level_boundary(self.h))
+
+    def commit(self, tiles: int) -> None:
+        check(api().w2d_level_commit(self.h, tiles))
+
+    def commit_strips(self, rows: int) -> None:
+        check(api().w2d_level_commit_strips(self.h, rows))
+
+    def wave(self) -> np.ndarray:
+        return self.b.wave()

@@ -37,13 +37,22 @@ def _from_upstream(a: np.ndarray, axis: int) -> np.ndarray:
     return out
 
 
+def make_stepper(backend, grid, num: Numerics) -> Stepper:
+    """The uniform step, or the level stepper when the numerics ask for local time stepping (model.py:23-29)."""
+    if not num.local_time_stepping:
+        return Stepper(backend, grid, num)
+    from wadi2d_amd.solver.level_stepper import LevelStepper
+
+    return LevelStepper(backend, grid, num)
+
+
 class Model:
     def __init__(self, tables, num: Numerics | None = None, backend: str = "auto", **options) -> None:
         self.tables = tables
         self.grid = tables.grid
         self.num = num or Numerics()
         self.backend = select_backend(backend, tables, NumericsRecord.from_config(self.num, self.grid.dx), **options)
-        self.stepper = Stepper(self.backend, self.grid, self.num)
+        self.stepper = make_stepper(self.backend, self.grid, self.num)
         self.boundary_tiles = np.zeros(0, np.int64)
         # the level and wet-share tables as grids, for starts given as mean depths
         self._hc = self.grid.from_tiles(tables.hc).astype(np.float64)
@@ -103,6 +112,7 @@ class Model:
         self.backend.set_boundary(bc)
         self.boundary_tiles = np.unique(np.asarray(bc.cell) // STRIDE2)
         self.stepper.keep_active[self.boundary_tiles] = True
+        self.stepper.pin(self.boundary_tiles)
         self.stepper.rebuild_active(retain=True)
 
     def read_cells(self, name: str, index) -> np.ndarray:
@@ -137,4 +147,4 @@ class Model:
         return int(local_index(t, tj + HALO, ti + HALO))
 
 
-__all__ = ["BoundaryCells", "Model", "tiles_touched"]
+__all__ = ["BoundaryCells", "Model", "make_stepper", "tiles_touched"]

@@ -38,6 +38,9 @@ class Stepper:
         self.keep_active = np.zeros(grid.ntiles, bool)
         self.idle = np.zeros(grid.ntiles, np.int32)
 
+    def pin(self, tiles: np.ndarray) -> None:
+        """Tiles that must see every smallest step: every tile does in the uniform step (stepper.py:49-50)."""
+
     def _wanted(self) -> np.ndarray:
         """Tiles that must be stepped now: each wet or held tile and its eight neighbours (water crosses less than a
         cell per step, so less than a tile between two checks)."""
