@@ -177,6 +177,8 @@ typedef struct w2d_member_state {
     int64_t nsteps;
     int32_t done, failed, refresh_due, dry;
     double dt_min; /* the smallest step length dt held after any step of this run (the run summary's dt_min) */
+    double q_peak, t_peak; /* the largest discharge out through the member's tally faces after a step of this run
+                            * (the FP64 sum of sign * q), and the time it was reached */
 } w2d_member_state;
 /* With the excess source (w2d_source_losses), a device-resident run takes (d0, d1, denom) per window, in the order of
  * the scales (3 nwindows doubles, nwindows = nscales), before w2d_run_setup: its windows' rates are formed on the

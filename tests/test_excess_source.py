@@ -47,6 +47,8 @@ def _valley(form: str):
     if form == "gridded":
         frames = frames_over(tables.grid)
         rain = Rain(tables, bset, None, curve_number=cn, frames=frames, area_factor=0.9)
+    elif form.startswith("gauges"):
+        rain = Rain(tables, bset, None, curve_number=cn, gauges=gauges_over(tables.grid, form.split("_")[1]))
     else:
         depths = rng.uniform(30.0, 80.0, (ny, nx)) if form == "depth_grid" else None
         storm = uniform_storm(1.0 if depths is not None else 50.0, 1800.0)
@@ -54,7 +56,23 @@ def _valley(form: str):
     return tables, bset, rain
 
 
-FORMS = ["uniform", "depth_grid", "gridded"]
+def gauges_over(grid, method: str, seed: int = 4):
+    """Five seeded gauges over and beside the grid, cumulative records every 10 min for 70 min (0-12 mm per period,
+    one gauge dry for a period), spread by nearest gauge ('thiessen') or inverse distance ('idw')."""
+    from wadi2d_amd.engine.gauges import INVERSE_DISTANCE, THIESSEN, GaugeStorm
+
+    ny, nx = grid.shape
+    rng = np.random.default_rng(seed)
+    xy = np.column_stack([grid.x0 + rng.uniform(-0.1, 1.1, 5) * nx * grid.dx,
+                          grid.y0 - rng.uniform(-0.1, 1.1, 5) * ny * grid.dx])  # fmt: skip
+    inc = rng.uniform(0.0, 12.0, (7, 5))
+    inc[3, 1] = 0.0
+    cumulative = np.vstack([np.zeros(5), np.cumsum(inc, axis=0)])
+    method = THIESSEN if method == "thiessen" else INVERSE_DISTANCE
+    return GaugeStorm(tuple(f"g{k}" for k in range(5)), xy, np.arange(8) * 10.0, cumulative, method)
+
+
+FORMS = ["uniform", "depth_grid", "gridded", "gauges_thiessen", "gauges_idw"]
 
 
 def _window(b, rain, t0, t1):
@@ -111,7 +129,7 @@ def _oracle_loop(tables, bset, rain, t_end):
 
 @pytest.mark.parametrize("form", FORMS)
 def test_the_host_loop_with_losses_is_the_oracle_bit_for_bit(form):
-    t_end = 2400.0 if form != "gridded" else 4800.0
+    t_end = 2400.0 if form in ("uniform", "depth_grid") else 4800.0
     tables, bset, rain = _valley(form)
     print(f"host loop vs oracle, excess form: {form}", flush=True)
     ref = _oracle_loop(tables, bset, rain, t_end)
@@ -132,9 +150,9 @@ def _plan(m, bset, rain, t_end):
     return MemberPlan(1, m.grid.ntiles, rain, bset, st.t, t_end, st.dt, st.nsteps, st.refresh_due, rain.state)
 
 
-@pytest.mark.parametrize("graph, form", [(0, "uniform"), (16, "uniform"), (16, "gridded")])
+@pytest.mark.parametrize("graph, form", [(0, "uniform"), (16, "uniform"), (16, "gridded"), (16, "gauges_idw")])
 def test_the_device_loop_with_losses_is_the_host_loop_bit_for_bit(graph, form):
-    t_end = 2400.0 if form != "gridded" else 4800.0
+    t_end = 2400.0 if form in ("uniform", "depth_grid") else 4800.0
     tables, bset, rain = _valley(form)
     host = Model(tables, None, "hip-exact")
     host.set_depth(np.zeros(tables.grid.shape))

@@ -672,6 +672,7 @@ struct Member {
     int stepping, refresh, refresh_due, new_window, dry, rebuild, check, done, failed;
     int nlist;  // the member's active tiles, kept at seg[tile0 .. tile0 + nlist)
     int par;  // the member's buffer parity (see Cur): a member that does not step keeps its own
+    double q_peak, t_peak;  // the largest discharge out through the member's tally faces after a step, and when
 };
 
 struct RunGlobals {
@@ -1958,6 +1959,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
                                                    const float* __restrict__ sign, double* __restrict__ total,
                                                    double* __restrict__ flow) {
     __shared__ double red_s[POST_THREADS / 64];
+    __shared__ double red_q[POST_THREADS / 64];
     Member& b = mem[blockIdx.x];
     if (!b.stepping) return;
     // the step's outputs: the next buffers of the parities it stepped at (thread 0 advances them below)
@@ -1977,12 +1979,16 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
             d[k] = pmax(d[ref[p]] + R(d0[p]), R(0));
         }
     }
+    double out_q = 0.0;  // this thread's share of the member's discharge out, summed in a fixed order
     for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) {
         const R qq = axis[q] ? s.qy[face[q]] : s.qx[face[q]];
         const double f = (double)(R(sign[q]) * qq);
         total[q] += b.dt_step * f;
         flow[q] = f;
+        out_q += f;
     }
+    for (int off = 32; off > 0; off >>= 1) out_q += __shfl_xor(out_q, off);
+    if ((threadIdx.x & 63) == 0) red_q[threadIdx.x >> 6] = out_q;
     // the largest wave speed of the member's tiles, gathered by the face kernel's atomic max
     double w = 0.0, rs = 0.0;
     const bool books_rain = b.new_window && !num.excess;  // the excess form's rain is booked by the pre kernel
@@ -2004,6 +2010,14 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     const long long before = b.nsteps;
     b.t += b.dt_step;
     b.nsteps = before + 1;
+    if (b.f1 > b.f0) {  // the peak outflow over the member's steps (the run summary's peak, monitor.py)
+        double q = red_q[0];
+        for (int w = 1; w < POST_THREADS / 64; ++w) q += red_q[w];
+        if (q > b.q_peak) {
+            b.q_peak = q;
+            b.t_peak = b.t;
+        }
+    }
     if (b.refresh) {
         if (!isfinite(w)) {
             b.failed = 1;
@@ -3800,6 +3814,8 @@ int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
         o.refresh_due = b.refresh_due;
         o.dry = b.dry;
         o.dt_min = b.dt_min;
+        o.q_peak = b.q_peak;
+        o.t_peak = b.t_peak;
         // rain credited pro rata in the current window (monitor.py:197-212)
         const double len = b.window_end - b.window_start;
         const double share = len > 0.0 ? (std::min(b.t, b.window_end) - b.window_start) / len : 0.0;

@@ -213,6 +213,8 @@ class BatchedResult:
     mass_error: list[float]
     launch: tuple[int, int] | None = None
     tuning: list | None = None
+    peak_q: list[float] = field(default_factory=list)  # per member: the largest outflow after a step, m3/s
+    t_peak: list[float] = field(default_factory=list)
 
 
 @dataclass
@@ -299,7 +301,8 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
         bal = run.balance(m, float(total[member_face == m].sum()))
         masses.append(bal.relative)
     return BatchedResult(wall, [int(s.nsteps) for s in state], [float(s.t) for s in state],
-                         [ens.member_field(m, dmax) for m in range(len(plans))], masses, launch, tuning)  # fmt: skip
+                         [ens.member_field(m, dmax) for m in range(len(plans))], masses, launch, tuning,
+                         [float(s.q_peak) for s in state], [float(s.t_peak) for s in state])  # fmt: skip
 
 
 @dataclass

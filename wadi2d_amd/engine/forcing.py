@@ -129,9 +129,10 @@ class Rain:
 
     * weight form: the curve times a per-cell weight (1 for a uniform storm). ``rate(t0, t1)`` is the window's mean
       rate per unit weight, ``scale``, times the weight, taken to float32 per cell.
-    * excess form (``curve_number``, ``depth_grid`` or ``frames`` given): the reference's own (forcing.py:644-678).
-      A cell's cumulative depth is the curve's times its depth factor (storm.depth_grid; the curve alone without
-      one), or with gridded rain its pixel's (``frames``, times ``area_factor``), its cumulative excess that depth
+    * excess form (``curve_number``, ``depth_grid``, ``frames`` or ``gauges`` given): the reference's own
+      (forcing.py:644-678). A cell's cumulative depth is the curve's times its depth factor (storm.depth_grid; the
+      curve alone without one), with gridded rain its pixel's (``frames``, times ``area_factor``), with gauge rain
+      its gauge weights times the gauges' depths (``gauges``, an engine.gauges.GaugeStorm), its cumulative excess that depth
       through the SCS loss of its curve number (NaN: none), and its rate the window's difference of excess over
       1000 (t1 - t0), taken to float32.
 
@@ -140,7 +141,7 @@ class Rain:
 
     def __init__(self, tables, boundary: BoundarySet, cumulative_mm, weight: np.ndarray | None = None,
                  curve_number: np.ndarray | None = None, depth_grid: np.ndarray | None = None,
-                 frames: RainFrames | None = None, area_factor: float = 1.0) -> None:  # fmt: skip
+                 frames: RainFrames | None = None, area_factor: float = 1.0, gauges=None) -> None:  # fmt: skip
         grid = tables.grid
         interior = grid.mask & ~boundary.mask
         self.jj, self.ii = np.nonzero(interior)
@@ -149,12 +150,21 @@ class Rain:
         self.cumulative_mm = cumulative_mm
         if weight is not None and (curve_number is not None or depth_grid is not None or frames is not None):
             raise ValueError("a weighted storm takes no curve numbers, depth grid or frames: give the excess form")
-        if frames is not None and depth_grid is not None:
-            raise ValueError("gridded rain brings its own depths: no depth grid with it")
+        if (frames is not None or gauges is not None) and depth_grid is not None:
+            raise ValueError("gridded or gauge rain brings its own depths: no depth grid with it")
+        if frames is not None and gauges is not None:
+            raise ValueError("rain comes from frames or from gauges, not both")
+        if weight is not None and gauges is not None:
+            raise ValueError("a weighted storm takes no gauges: give the excess form")
         self.pixels = None
         if frames is not None:
             x, y = grid.xy_of(self.jj, self.ii)
             self.pixels = CellPixels(frames, x, y, area_factor)
+        if gauges is not None:  # gauge rain through the same per-cell index (engine/gauges.py)
+            from wadi2d_amd.engine.gauges import GaugeCells
+
+            x, y = grid.xy_of(self.jj, self.ii)
+            self.pixels = GaugeCells(gauges, x, y)
         self.weight = None if weight is None else np.asarray(weight, np.float64)[self.jj, self.ii]
         self.cn = None if curve_number is None else np.asarray(curve_number, np.float64)[self.jj, self.ii]
         self.depths = None if depth_grid is None else np.asarray(depth_grid, np.float64)[self.jj, self.ii]

@@ -49,7 +49,7 @@ class RunNumerics(ctypes.Structure):
 
 class MemberState(ctypes.Structure):
     _fields_ = [("t", D), ("dt", D), ("rain", D), ("nsteps", I64), ("done", I32), ("failed", I32),
-                ("refresh_due", I32), ("dry", I32), ("dt_min", D)]  # fmt: skip
+                ("refresh_due", I32), ("dry", I32), ("dt_min", D), ("q_peak", D), ("t_peak", D)]  # fmt: skip
 
 
 _SIGS = {
