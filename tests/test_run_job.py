@@ -117,6 +117,7 @@ def test_a_reference_shaped_job_with_inert_sections_loads(job_folder):
         (lambda d: d["boundaries"].append({"kind": "flow", "coordinates": [[0, 0], [1, 1]], "name": "in",
                                            "flow": {"t": [0], "q": [1]}}), "flow"),  # fmt: skip
         (lambda d: d["storm"].update(kind="gridded"), "gridded"),
+        (lambda d: d["storm"].update(kind="gauges"), "gauges"),
         (lambda d: d["numerics"].update(local_time_stepping=True), "numerics.local_time_stepping"),
         (lambda d: d.update(breaches=[{"name": "b"}]), "breaches"),
         (lambda d: d["grid"].update(domain="outline.gpkg"), "grid.domain"),
@@ -305,6 +306,8 @@ def _oracle_run(cfg, full: bool = False):
     tables = build_grid(cfg)
     bset = boundary_set(boundary_list(cfg, tables), tables)
     rain = build_rain(cfg, tables, bset, storm_curve(cfg))
+    if cfg.numerics.local_time_stepping:
+        from oracle.levels import OracleLevelModel as OracleModel  # noqa: F811
     m = OracleModel(tables, ONumerics.like(cfg.numerics), prec=4, threads=4)
     m.set_depth(initial_depth(cfg, tables))
     if bset.size:
@@ -389,6 +392,28 @@ def test_a_job_runs_end_to_end_bit_for_bit_the_oracle(job_folder):
     assert (job_folder / "out" / "series" / "lines_gap.csv").exists()
     maps = sorted((job_folder / "out" / "results" / "maps").iterdir())
     assert len(maps) >= 2 and (maps[0] / "depth.npy").exists()
+
+
+@pytest.mark.gpu
+@pytest.mark.skip(reason="its one GPU run faulted the device (an illegal address reported at the cycle's wave read); "
+                  "the cause is not found, so a job refuses local time stepping and this case stays off the GPU")
+def test_a_job_with_local_time_stepping_runs_bit_for_bit_the_oracle_level_stepper(job_folder):
+    """numerics.local_time_stepping: the run takes the level stepper (model.py:23-29), its boundary tiles pinned and
+    its tally counted every smallest step; the maxima equal the oracle's level stepper's bit for bit."""
+    from wadi2d_amd.run import run_job
+
+    d = json.loads((job_folder / "job.json").read_text())
+    d.setdefault("numerics", {}).update(local_time_stepping=True, lts_levels=2)
+    (job_folder / "job.json").write_text(json.dumps(d))
+    cfg = jobmod.load_job(job_folder / "job.json")
+    s = run_job(cfg, job_folder / "out", backend="hip-exact")
+    print(f"LTS run: exit {s.exit_code} steps {s.steps} mass {s.balance.relative:.2e}", flush=True)
+    assert s.exit_code == 0, s.message
+    dmax, _bal, steps = _oracle_run(cfg)
+    got = np.load(job_folder / "out" / "results" / "maxima" / "depth.npy")
+    assert s.steps == steps
+    assert np.array_equal(got, dmax.astype(np.float32))
+    assert abs(s.balance.relative) <= 1e-6
 
 
 def _loss_job(folder: Path, holes: bool = False, gridded: bool = False) -> dict:
