@@ -435,10 +435,21 @@ def learned(args) -> int:
         jp, _ = coarse_run(cs, col.coarse, None)
         jl, _ = coarse_run(cs, col.coarse, model)
         mid, mrun = middle_runs(pairs, col.coarse, 64, 2, 5400.0)
-        check = {"plain_20m": errors(jp.max_depth, col.dmax, col.coarse).__dict__,
-                 "learned_20m": errors(jl.max_depth, col.dmax, col.coarse).__dict__,
-                 "plain_10m": errors(mid.dmax, col.dmax, col.coarse).__dict__,
-                 "wall_s": {"plain_20m": jp.wall_s, "learned_20m": jl.wall_s, "plain_10m": mrun.wall_s}}  # fmt: skip
+        e_plain, e_learned = errors(jp.max_depth, col.dmax, col.coarse), errors(jl.max_depth, col.dmax, col.coarse)
+        e_mid = errors(mid.dmax, col.dmax, col.coarse)
+        # the plain way at a finer grid, member after member: what the reference would run for the accuracy the
+        # closure buys on the coarse grid
+        ser10 = run_serial(mid.ens, "hip", None, 1, log=log)
+        ser20 = run_serial(Ensemble.from_scenarios(cs), "hip", None, 1, log=log)
+        check = {"plain_20m": e_plain.__dict__, "learned_20m": e_learned.__dict__, "plain_10m": e_mid.__dict__,
+                 "wall_s": {"plain_20m": jp.wall_s, "learned_20m": jl.wall_s, "plain_10m": mrun.wall_s,
+                            "plain_10m_serial": ser10.wall_s, "plain_20m_serial": ser20.wall_s},
+                 "matched_accuracy": {
+                     "learned_20m_batched_rmse_m": e_learned.rmse, "plain_10m_serial_rmse_m": e_mid.rmse,
+                     "plain_20m_serial_rmse_m": e_plain.rmse,
+                     "speedup_learned_batched_vs_plain_10m_serial": ser10.wall_s / jl.wall_s,
+                     "speedup_learned_batched_vs_plain_20m_serial": ser20.wall_s / jl.wall_s,
+                     "target_rmse_m": 0.02, "target_met": bool(e_learned.rmse <= 0.02)}}  # fmt: skip
     out = {
         "metric": "ensemble member-hours per wall-hour, batched with the learned closure",
         "value": rate,

@@ -1,9 +1,12 @@
 // The learned face closure's network, 16 -> 64 -> 64 -> 2 with ReLU, on CDNA matrix cores.
 //
-// Weights are BF16, stored transposed (out x in, the layout a B operand of v_mfma_f32_16x16x16_bf16 reads with one
-// 8-byte access per lane); biases and accumulation are FP32. Activations between layers are rounded to BF16. A wave
-// evaluates 16 samples: per 16 x 16 output tile, lane l holds A[row l%16][k 4(l/16)..+3] and B[k 4(l/16)..+3][col
-// l%16], and receives D[row 4(l/16)+j][col l%16], j = 0..3.
+// Weights are BF16, stored transposed (out x in: the rows a B operand reads are contiguous); biases and
+// accumulation are FP32. Activations between layers are rounded to BF16. A wave evaluates 16 samples. The first
+// layer (K = 16) runs on v_mfma_f32_16x16x16_bf16: lane l holds A[row l%16][k 4(l/16)..+3] and B[k 4(l/16)..+3][col
+// l%16] (one 8-byte access each). The two K = 64 layers run on gfx950's v_mfma_f32_16x16x32_bf16, two issues per
+// output tile instead of four: lane l holds A[row l%16][k 8(l/16)..+7] and the same k of B (one 16-byte access
+// each; any assignment of k to lanes gives the same product so long as A and B share it). Either way lane l receives
+// D[row 4(l/16)+j][col l%16], j = 0..3.
 //
 // FP32 parameter vector (the optimiser's master copy), in this order:
 //   W1 [64][16], b1 [64], W2 [64][64], b2 [64], W3 [2][64], b3 [2]            (MLP_NPARAM floats)
@@ -23,6 +26,7 @@ constexpr int BF_W1 = 0, BF_W2 = BF_W1 + HID * IN, BF_W3 = BF_W2 + HID * HID, NB
 constexpr int WAVE_SAMPLES = 16;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> BF16 bits, round to nearest even (finite inputs; the network's values are bounded).
@@ -69,6 +73,9 @@ __device__ inline bf16x4 ld4(const uint16_t* p) {
     return r;
 }
 
+// Eight BF16 values from a 16-byte aligned address.
+__device__ inline bf16x8 ld8(const uint16_t* p) { return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p)); }
+
 // One wave, 16 samples: x [16][IN] BF16 (LDS) -> h1, h2 [16][HID] BF16 (LDS, post-ReLU) and y [16][OUT] FP32
 // (LDS). The wave's lanes all take part; the caller orders LDS between waves.
 __device__ inline void forward_wave(const Net& n, const uint16_t* x, uint16_t* h1, uint16_t* h2, float* y) {
@@ -88,14 +95,15 @@ __device__ inline void forward_wave(const Net& n, const uint16_t* x, uint16_t* h
     }
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
+    const int k8 = 8 * (lane >> 4);
 #pragma unroll
     for (int nt = 0; nt < HID / 16; ++nt) {
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int kt = 0; kt < HID / 16; ++kt)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(h1 + r * HID + kt * 16 + k0),
-                                                            ld4(n.w + BF_W2 + (nt * 16 + r) * HID + kt * 16 + k0), acc,
-                                                            0, 0, 0);
+        for (int kt = 0; kt < HID / 32; ++kt)
+            acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(h1 + r * HID + kt * 32 + k8),
+                                                          ld8(n.w + BF_W2 + (nt * 16 + r) * HID + kt * 32 + k8), acc,
+                                                          0, 0, 0);
         const int col = nt * 16 + r;
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
@@ -107,9 +115,9 @@ __device__ inline void forward_wave(const Net& n, const uint16_t* x, uint16_t* h
     __builtin_amdgcn_wave_barrier();
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int kt = 0; kt < HID / 16; ++kt)
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(h2 + r * HID + kt * 16 + k0),
-                                                        ld4(n.w + BF_W3 + r * HID + kt * 16 + k0), acc, 0, 0, 0);
+    for (int kt = 0; kt < HID / 32; ++kt)
+        acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ld8(h2 + r * HID + kt * 32 + k8),
+                                                      ld8(n.w + BF_W3 + r * HID + kt * 32 + k8), acc, 0, 0, 0);
     if (r < OUT)
 #pragma unroll
         for (int j = 0; j < 4; ++j) y[(k0 + j) * OUT + r] = acc[j] + n.b3[r];
