@@ -345,3 +345,45 @@ def test_the_hip_path_sleeps_and_gives_the_all_tiles_run(backend):
     print(f"{backend}: awake tiles {awake} of {a.grid.ntiles - 1}", flush=True)
     assert min(awake) <= 8 and awake[0] < awake[-1]
     assert b.backend.nact == b.grid.ntiles - 1
+
+
+# ---- the running maxima, against their definitions written on the (ny, nx) grid
+
+
+def test_the_oracle_maxima_are_their_definitions_on_the_grid():
+    """engine/maxima.py:29-51 written a second way: on whole (ny, nx) fields, the speed at a cell centre from the mean
+    of its west and east face velocities (the east face is the next column's west face, zero past the grid) and of
+    its north and south ones; a planted mistake (no half in the centre speed) differs."""
+    from oracle.oracle import Maxima, from_state
+    from tests.cases import rough_dam_break
+
+    tables, h0 = rough_dam_break()
+    m = OracleModel(tables, ONumerics.like(ONumerics()), prec=8, threads=THREADS)
+    m.set_depth(h0)
+    mx = Maxima(m, arrive=0.1, deep=0.05)
+    g = tables.grid
+    want = {"depth": np.zeros(g.shape), "mean_depth": np.zeros(g.shape), "speed": np.zeros(g.shape),
+            "hazard": np.zeros(g.shape), "arrival": np.full(g.shape, -1.0), "planted": np.zeros(g.shape)}  # fmt: skip
+
+    def sample(st):
+        mx.update(st.t)
+        d, h, u, v = (m.field(f) for f in ("d", "h", "u", "v"))
+        east = np.concatenate([u[:, 1:], np.zeros((u.shape[0], 1))], axis=1)
+        south = np.concatenate([v[1:, :], np.zeros((1, v.shape[1]))], axis=0)
+        sp = np.hypot(0.5 * (u + east), 0.5 * (v + south))
+        want["depth"] = np.maximum(want["depth"], d)
+        want["mean_depth"] = np.maximum(want["mean_depth"], h)
+        deep = d > 0.05
+        want["speed"] = np.where(deep, np.maximum(want["speed"], sp), want["speed"])
+        want["hazard"] = np.where(deep, np.maximum(want["hazard"], sp * d), want["hazard"])
+        want["arrival"] = np.where((want["arrival"] < 0) & (d >= 0.1), st.t, want["arrival"])
+        west_only = np.hypot(u, v)  # the planted mistake: the west and north faces' speed, no centre mean
+        want["planted"] = np.where(deep, np.maximum(want["planted"], west_only), want["planted"])
+
+    m.advance(20.0, sample)
+    mask = g.mask
+    for name, arr in zip(Maxima.NAMES, mx.arrays, strict=True):
+        got = from_state(g, arr)
+        assert np.allclose(got[mask], want[name][mask], rtol=1e-12, atol=1e-12), name
+    assert (want["speed"][mask] > 0.5).any() and (want["arrival"][mask] > 0).any()
+    assert not np.allclose(from_state(g, mx.arrays[2])[mask], want["planted"][mask], rtol=1e-3)
