@@ -566,14 +566,15 @@ def _same_outputs(a: Path, b: Path) -> None:
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("losses", [False, True])
+@pytest.mark.parametrize("losses", [False, True, "gridded"])
 def test_the_device_loop_runs_a_job_bit_for_bit_the_host_loop(job_folder, losses):
     """``--loop device``: the steps on the device in chunks between outputs give the host loop's series, maps and
-    maxima to the bit, the same step count, smallest dt and peaks, the same books to rounding."""
+    maxima to the bit, the same step count, smallest dt and peaks, the same books to rounding (uniform rain, curve
+    numbers on a depth grid, gridded rain frames)."""
     from wadi2d_amd.run import run_job
 
     if losses:
-        _loss_job(job_folder)
+        _loss_job(job_folder, gridded=losses == "gridded")
     cfg = jobmod.load_job(job_folder / "job.json")
     host = run_job(cfg, job_folder / "host", backend="hip-exact")
     dev = run_job(cfg, job_folder / "dev", backend="hip-exact", loop="device")
@@ -634,6 +635,48 @@ def test_every_boundary_kind_and_storm_kind_runs_and_a_collapsing_step_ends_in_e
     events = [json.loads(x) for x in (job_folder / "fail" / "progress.jsonl").read_text().splitlines()]
     kinds = [e["ev"] for e in events]
     assert "error" in kinds and kinds[-1] == "done" and events[-1]["code"] == 5 and events[-1]["partial"]
+
+
+@pytest.mark.gpu
+def test_a_collapsing_step_on_the_device_loop_ends_as_the_host_loop_does(job_folder):
+    """Exit 5 on ``--loop device`` (stepper.py:117-124, run.py:1749-1757): with the minimum step set just above
+    the smallest step a finished run took, the run collapses at that step, somewhere inside a chunk of device
+    steps. Both loops stop at the same step with the same message; the failing step is not booked (no tally, no
+    smallest dt), the failed member stays listed so the final maxima sample covers its tiles, and the series,
+    maxima and books equal the host loop's. Stage, normal and free boundaries under a hyetograph storm."""
+    from wadi2d_amd.run import run_job
+
+    xe, yn = X0 + NX * DX, Y0 + NY * DX
+    d = job_dict(duration=300.0)
+    d["storm"] = {"kind": "hyetograph", "hyetograph_min": [2.0, 4.0], "hyetograph_mm": [10.0, 20.0]}
+    d["boundaries"] = [
+        {"kind": "free", "coordinates": [[xe - 1.0, yn], [xe - 1.0, Y0 + 320.0]], "name": "east top"},
+        {"kind": "normal", "slope": 0.01, "coordinates": [[xe - 1.0, Y0 + 310.0], [xe - 1.0, Y0]], "name": "east"},
+        {"kind": "stage", "stage": 100.5, "coordinates": [[X0 + 1.0, Y0 + 100.0], [X0 + 1.0, Y0]], "name": "west"},
+    ]
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    ok = run_job(jobmod.load_job(job_folder / "job.json"), job_folder / "ok", backend="hip-exact")
+    assert ok.exit_code == 0, ok.message
+    d["numerics"] = {"dt_min": ok.dt_min * (1.0 + 1e-6)}
+    (job_folder / "job.json").write_text(json.dumps(d), encoding="utf-8")
+    cfg = jobmod.load_job(job_folder / "job.json")
+    host = run_job(cfg, job_folder / "host", backend="hip-exact")
+    dev = run_job(cfg, job_folder / "dev", backend="hip-exact", loop="device")
+    print(f"collapse: finished run {ok.steps} steps, dt_min {ok.dt_min:.6e}; host exit {host.exit_code} at step "
+          f"{host.steps}, device exit {dev.exit_code} at step {dev.steps}", flush=True)
+    assert host.exit_code == 5 and "below the minimum" in host.message
+    assert dev.exit_code == 5 and dev.message == host.message
+    assert dev.steps == host.steps and dev.t_end == host.t_end and dev.dt_min == host.dt_min
+    assert 0 < host.steps < ok.steps
+    for name in ("depth", "mean_depth", "speed", "hazard", "arrival"):
+        x = np.load(job_folder / "host" / "results" / "maxima" / f"{name}.npy")
+        y = np.load(job_folder / "dev" / "results" / "maxima" / f"{name}.npy")
+        assert np.array_equal(x, y, equal_nan=True), name
+    for f in ("lines_gap.csv", "points_p1.csv"):
+        assert (job_folder / "host" / "series" / f).read_text() == (job_folder / "dev" / "series" / f).read_text(), f
+    bh, bd = host.balance, dev.balance
+    assert bd.storage == bh.storage
+    assert bd.outflow == pytest.approx(bh.outflow, rel=1e-12) and bd.rain == pytest.approx(bh.rain, rel=1e-12)
 
 
 # ---- the run folder's files

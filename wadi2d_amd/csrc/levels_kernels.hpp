@@ -74,18 +74,20 @@ __device__ __forceinline__ R level_face(const LevelArgs<R>& L, const Cur<R>& s, 
     if (xaxis) {
         const R ef = R(fabsf(e.x));
         const R cw = signbit(e.x) ? R(A.st.cwv[c].x) : R(0);
-        w = face<R, FAST>(fin, Lx, s.mx[c], A.st.sx[c], ef, cw, A.st.zcx, A.st.ftabx + c, A.st.ncell, A.st.last,
-                          A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c, s.om, west_of(t, j, i, nb), c, L.dt,
-                          want_wave, A.P, un, mn, qn);
+        const unsigned k = west_of(t, j, i, nb);
+        w = face<R, FAST>(fin, Lx, s.mx[c], A.st.sx[c], ef, cw, R(A.st.zcx[k]), R(A.st.zcx[c]), A.st.ftabx + c,
+                          A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
+                          want_wave ? s.om[k] : R(0), want_wave ? s.om[c] : R(0), L.dt, want_wave, A.P, un, mn, qn);
         s.u1[c] = un;
         mx1[c] = mn;
         s.qx1[c] = qn;
     } else {
         const R ef = R(fabsf(e.y));
         const R cw = signbit(e.y) ? R(A.st.cwv[c].y) : R(0);
-        w = face<R, FAST>(fin, Lx, s.my[c], A.st.sy[c], ef, cw, A.st.zcy, A.st.ftaby + c, A.st.ncell, A.st.last,
-                          A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c, s.om, north_of(t, j, i, nb), c, L.dt,
-                          want_wave, A.P, un, mn, qn);
+        const unsigned k = north_of(t, j, i, nb);
+        w = face<R, FAST>(fin, Lx, s.my[c], A.st.sy[c], ef, cw, R(A.st.zcy[k]), R(A.st.zcy[c]), A.st.ftaby + c,
+                          A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
+                          want_wave ? s.om[k] : R(0), want_wave ? s.om[c] : R(0), L.dt, want_wave, A.P, un, mn, qn);
         s.v1[c] = un;
         my1[c] = mn;
         s.qy1[c] = qn;

@@ -478,12 +478,14 @@ struct FaceIn {
 // statement order of solver/cpu/kernels.py:86-219. Returns the wave speed. LEARNED: the learned closure's
 // correction of this face, `corr` = (conveyance factor on T, rise of the face's inverts), is applied (no counterpart
 // in the reference; the plain path never takes this branch).
+// `zc_up`, `zc_own`: the centre-line beds of the face's two cells (west or north, own); `om_up`, `om_own`: their
+// wet plan shares, read only on a wave step. The caller loads them before the face's upwind side is known, so no
+// load waits on it.
 template <class R, bool FAST, bool LEARNED = false>
-__device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st, R ef, R cw, const float* zc,
+__device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st, R ef, R cw, R zc_up, R zc_own,
                                   const float4* __restrict__ ftab, size_t plane, int last, float4* cache,
-                                  unsigned char* tag, const R* __restrict__ om, unsigned k_up, unsigned k_own, R dt,
-                                  bool want_wave, const Params& P, R& u_out, R& m_out, R& q_out,
-                                  float2 corr = float2{1.f, 0.f}) {
+                                  unsigned char* tag, R om_up, R om_own, R dt, bool want_wave, const Params& P,
+                                  R& u_out, R& m_out, R& q_out, float2 corr = float2{1.f, 0.f}) {
     const int S = in.S, Q = in.P;
     const R* u = in.su;
     const R g = R(P.g), dx = R(P.dx);
@@ -512,7 +514,7 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     }
     R eface = eup;
     if (P.carry && cw == R(0)) {
-        const R dz = from_left ? R(zc[k_up]) - fl : R(zc[k_own]) - fr;
+        const R dz = from_left ? zc_up - fl : zc_own - fr;
         if (dz > R(1e-6)) {
             R phi = dv<FAST>(eup - R(2) * dz, R(2) * dz);
             phi = phi < R(0) ? R(0) : (phi > R(1) ? R(1) : phi);
@@ -625,9 +627,9 @@ __device__ __forceinline__ R face(const FaceIn<R>& in, int L, R m_own, float4 st
     const R dup = from_left ? dL : dR;
     const R h_cfl = R(P.h_cfl);
     if (dup <= h_cfl) return R(0);
-    R plan = from_left ? om[k_up] : om[k_own];
+    R plan = from_left ? om_up : om_own;
     const R dother = from_left ? dR : dL;
-    if (dother > h_cfl) plan = fmn<FAST>(plan, from_left ? om[k_own] : om[k_up]);
+    if (dother > h_cfl) plan = fmn<FAST>(plan, from_left ? om_own : om_up);
     const R wave = sq<FAST>(fmx<FAST>(dv<FAST>(g * A * rho, fmx<FAST>(plan, R(P.a_min)) * dx), R(0)));
     return fabs(un) + fmn<FAST>(R(P.lead) * fabs(un - uf), R(2) * wave) + wave;
 }
@@ -792,14 +794,22 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
             mxo = R(0.5) * (s.h_next[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
             myo = R(0.5) * (s.h_next[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
         }
-        const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), efx, cwx, A.st.zcx, A.st.ftabx + c,
+        const unsigned kw = west_of(t, j, i, nb), kn = north_of(t, j, i, nb);
+        const R zxw = R(A.st.zcx[kw]), zxo = R(A.st.zcx[c]), zyn = R(A.st.zcy[kn]), zyo = R(A.st.zcy[c]);
+        R omw = R(0), omn = R(0), omo = R(0);
+        if (want_wave) {
+            omw = s.om[kw];
+            omn = s.om[kn];
+            omo = s.om[c];
+        }
+        const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), efx, cwx, zxw, zxo, A.st.ftabx + c,
                                             A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
-                                            s.om, west_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cx);
+                                            omw, omo, dt, want_wave, A.P, un, mn, qn, cx);
         st_nt(un, s.u1 + c);
         st_nt(qn, s.qx1 + c);
-        const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), efy, cwy, A.st.zcy, A.st.ftaby + c,
+        const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), efy, cwy, zyn, zyo, A.st.ftaby + c,
                                             A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
-                                            s.om, north_of(t, j, i, nb), c, dt, want_wave, A.P, un, mn, qn, cy);
+                                            omn, omo, dt, want_wave, A.P, un, mn, qn, cy);
         st_nt(un, s.v1 + c);
         st_nt(qn, s.qy1 + c);
         const R m = wx > wy ? wx : wy;
@@ -1436,6 +1446,9 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.rebuild = 0;
     if (b.done) return;
     if (b.t >= b.t_stop) return;  // held where the host asked (its end too): it reads this state before the next step
+    // a failed member the host holds (a finite stop time: a job's chunks) stays listed as it failed, so the host
+    // reads the failing step's state and samples its maxima over the same tiles as the host loop would
+    if (b.failed && b.t_stop < INFINITY) return;
     if (b.failed || b.t >= b.t_end - 1e-9) {
         b.done = 1;
         b.rebuild = 3;
@@ -1979,18 +1992,22 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
             d[k] = pmax(d[ref[p]] + R(d0[p]), R(0));
         }
     }
+    // the step's wave speed and whether it fails (stepper.py:117-124), known to every thread before the books: a
+    // failing step books nothing, as the host loop raises before its after-step work (tally, smallest dt, peak)
+    const double w = b.refresh ? wave_value<R>(member_wave[blockIdx.x]) : 0.0;
+    const double dt_next = b.refresh ? fmin(num.cfl * num.dx / fmax(w, num.wave_floor), num.dt_max) : b.dt;
+    const int failing = !b.refresh ? 0 : !isfinite(w) ? 1 : dt_next < num.dt_min ? 2 : 0;
     double out_q = 0.0;  // this thread's share of the member's discharge out, summed in a fixed order
     for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) {
         const R qq = axis[q] ? s.qy[face[q]] : s.qx[face[q]];
         const double f = (double)(R(sign[q]) * qq);
-        total[q] += b.dt_step * f;
+        if (!failing) total[q] += b.dt_step * f;
         flow[q] = f;
         out_q += f;
     }
     for (int off = 32; off > 0; off >>= 1) out_q += __shfl_xor(out_q, off);
     if ((threadIdx.x & 63) == 0) red_q[threadIdx.x >> 6] = out_q;
-    // the largest wave speed of the member's tiles, gathered by the face kernel's atomic max
-    double w = 0.0, rs = 0.0;
+    double rs = 0.0;
     const bool books_rain = b.new_window && !num.excess;  // the excess form's rain is booked by the pre kernel
     if (books_rain) {
         for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x)
@@ -2001,7 +2018,6 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     }
     __syncthreads();  // every thread has read the parities before thread 0 moves them on
     if (threadIdx.x != 0) return;
-    if (b.refresh) w = wave_value<R>(member_wave[blockIdx.x]);
     if (books_rain) {
         rs = red_s[0];
         for (int q = 1; q < POST_THREADS / 64; ++q) rs += red_s[q];
@@ -2010,7 +2026,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
     const long long before = b.nsteps;
     b.t += b.dt_step;
     b.nsteps = before + 1;
-    if (b.f1 > b.f0) {  // the peak outflow over the member's steps (the run summary's peak, monitor.py)
+    if (b.f1 > b.f0 && !failing) {  // the peak outflow over the member's steps (the run summary's peak, monitor.py)
         double q = red_q[0];
         for (int w = 1; w < POST_THREADS / 64; ++w) q += red_q[w];
         if (q > b.q_peak) {
@@ -2018,15 +2034,9 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
             b.t_peak = b.t;
         }
     }
-    if (b.refresh) {
-        if (!isfinite(w)) {
-            b.failed = 1;
-        } else {
-            b.dt = fmin(num.cfl * num.dx / fmax(w, num.wave_floor), num.dt_max);
-            if (b.dt < num.dt_min) b.failed = 2;
-        }
-    }
-    b.dt_min = fmin(b.dt_min, b.dt);
+    if (failing != 1) b.dt = dt_next;  // a non-finite speed leaves dt as it was (stepper.py:117-119)
+    b.failed = failing;
+    if (!failing) b.dt_min = fmin(b.dt_min, b.dt);
     if (before % num.active_every == 0) b.check = 1;
     b.par = 1 - b.par;
 }

@@ -105,8 +105,8 @@ class DeviceRun:
         return self.state
 
     def run(self, chunk: int = 256, graph_steps: int = 64, on_chunk=None) -> list:
-        """Step until every member is done (or failed)."""
-        while not all(s.done for s in self.state):
+        """Step until every member is done or has failed (a failed member held at a stop time is not retired)."""
+        while not all(s.done or s.failed for s in self.state):
             self.steps(chunk, graph_steps)
             if on_chunk is not None:
                 on_chunk(self.state)
