@@ -74,6 +74,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=float, default=0.0, help="MiB of a plain device copy to time first")
     ap.add_argument("--loop", choices=("device", "host"), default="device")
     ap.add_argument("--graph", type=int, default=64, help="steps per captured graph (device loop)")
+    ap.add_argument("--pipeline", choices=("on", "off"), default="on",
+                    help="device loop: the pipelined step (a step's cell update made with the next step's faces)")
     ap.add_argument("--face-cache", choices=("on", "off"), default="on", help="per-face row cache of the face tables")
     ap.add_argument("--faces-threads", type=int, choices=(256, 512), default=512, help="face kernel workgroup size")
     ap.add_argument("--cells-threads", type=int, choices=(256, 512), default=256, help="cell kernel workgroup size")
@@ -166,7 +168,7 @@ def main() -> int:
 
         plan = MemberPlan(1, tables.grid.ntiles, rain, bset, st.t, scen.duration_s, st.dt, st.nsteps,
                           st.refresh_due, rain.state)  # fmt: skip
-        run = DeviceRun(b, [plan], scen.numerics, idle=st.idle)
+        run = DeviceRun(b, [plan], scen.numerics, idle=st.idle, pipeline=args.pipeline == "on")
         state = run.state[0]
         while state.t < hot - 1e-9:
             state = run.steps(256, args.graph)[0]
@@ -197,7 +199,7 @@ def main() -> int:
             b.maxima_set(0.1, 0.05)
             plan4 = MemberPlan(1, tables.grid.ntiles, rain, bset, state.t, scen.duration_s, state.dt, state.nsteps,
                                True, rain.state)  # fmt: skip
-            run4 = DeviceRun(b, [plan4], scen.numerics, maxima=4, idle=b.run_idle())
+            run4 = DeviceRun(b, [plan4], scen.numerics, maxima=4, idle=b.run_idle(), pipeline=args.pipeline == "on")
             run4.steps(args.warmup, args.graph)
             t1 = time.perf_counter()
             s4 = run4.steps(args.maxima_steps, args.graph)[0]

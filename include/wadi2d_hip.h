@@ -179,6 +179,7 @@ typedef struct w2d_member_state {
     double dt_min; /* the smallest step length dt held after any step of this run (the run summary's dt_min) */
     double q_peak, t_peak; /* the largest discharge out through the member's tally faces after a step of this run
                             * (the FP64 sum of sign * q), and the time it was reached */
+    int32_t pipelined;     /* the member takes the pipelined step (w2d_run_pipeline) */
 } w2d_member_state;
 /* With the excess source (w2d_source_losses), a device-resident run takes (d0, d1, denom) per window, in the order of
  * the scales (3 nwindows doubles, nwindows = nscales), before w2d_run_setup: its windows' rates are formed on the
@@ -205,6 +206,14 @@ int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
  * an output time, as the host loop's step hook does); +inf lets it run on. Steps enqueued while a member is held
  * change nothing of it, whatever the other members do. */
 int w2d_run_pause(w2d_handle* h, const double* t_stop, int32_t M);
+/* The pipelined step of device-resident runs (no counterpart; on by default, taken at the next w2d_run_setup): a
+ * step whose next step keeps the active list and reads no wet flag leaves its cell update to that step's kernel,
+ * which makes it and then the face step in one pass over each tile, recomputing the update of its west column and
+ * north row instead of reading the level and limited discharges back from memory. Every field, maximum and book is
+ * the two-kernel step's, bit for bit; run_steps returns with no update waiting. Runs with the learned closure, and
+ * members with a following boundary cell (kind 2) whose reference lies outside what a tile updates, take the
+ * two-kernel step. on: 0 or 1. */
+int w2d_run_pipeline(w2d_handle* h, int32_t on);
 int w2d_run_idle(w2d_handle* h, int32_t* out);
 /* The active list as the device last left it (out may be null to ask for the count). */
 int w2d_active_tiles(w2d_handle* h, int32_t* out, int32_t* n);

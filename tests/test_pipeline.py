@@ -1,0 +1,101 @@
+"""The pipelined step of device-resident runs against the two-kernel step, through the C-ABI.
+
+With the pipelined step a step's cell update is made by the next step's kernel, together with that step's faces, and
+each tile recomputes the update of its west column and north row instead of reading it back. Nothing of the scheme
+changes (cells.cu:4-111, faces.cu:244-271 in the reference's order, solver/stepper.py:101-129 around them): every
+field, running maximum, tally total, clock and book of a run must be the two-kernel run's, bit for bit, whatever the
+run asks of it (sleeping tiles, boundary cells, maxima every step or every fourth, graphs or single launches, a member
+held while another steps)."""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from tests.test_device_run import _device_plan, _valley_setup
+from wadi2d_amd.engine.device_run import DeviceRun
+from wadi2d_amd.engine.driver import Driver
+from wadi2d_amd.engine.ensemble import Ensemble, MemberSpec, setup_batched
+from wadi2d_amd.solver.state import from_state
+
+FIELDS = ("h", "d", "u", "v", "mx", "my", "qx", "qy", "res", "om", "src")
+STATE = ("t", "dt", "rain", "nsteps", "done", "failed", "dt_min", "q_peak", "t_peak")
+
+
+def _valley_run(backend, pipeline, maxima, t_end, chunks):
+    _, bset, rain, m = _valley_setup(backend)
+    Driver(m, bset, rain, maxima_every=max(maxima, 1))
+    run = DeviceRun(m.backend, [_device_plan(m, bset, rain, t_end)], m.num, maxima=maxima, idle=m.stepper.idle,
+                    pipeline=pipeline)  # fmt: skip
+    for n, graph in chunks:
+        s = run.steps(n, graph)
+    return m, s[0]
+
+
+def _same_state(a, b):
+    for k in STATE:
+        assert getattr(a, k) == getattr(b, k), k
+
+
+def _same_model(ma, mb, grid):
+    for f in FIELDS:
+        x, y = ma.field(f), mb.field(f)
+        assert np.array_equal(x, y), f"{f}: {np.abs(x - y).max()}"
+    for which in range(5):
+        x = from_state(grid, ma.backend.maxima_get(which))
+        y = from_state(grid, mb.backend.maxima_get(which))
+        assert np.array_equal(x, y, equal_nan=True), which
+    assert np.array_equal(ma.backend.tally_totals(), mb.backend.tally_totals())
+    assert np.array_equal(np.sort(ma.backend.active_tiles), np.sort(mb.backend.active_tiles))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend,maxima", [("hip-exact", 1), ("hip-exact", 4), ("hip-f64", 4), ("hip", 4)])
+def test_pipelined_run_is_the_two_kernel_run_bit_for_bit(backend, maxima):
+    """A rained valley draining through a free outflow: tiles wake and sleep, windows open every 30 s, the run ends
+    with its member retired; chunks in graphs and one launch at a time."""
+    chunks = [(100, 16), (37, 0), (2000, 64), (5, 0), (400, 16)]
+    a, sa = _valley_run(backend, True, maxima, 2400.0, chunks)
+    b, sb = _valley_run(backend, False, maxima, 2400.0, chunks)
+    print(f"{backend} maxima {maxima}: {sa.nsteps} steps, pipelined {sa.pipelined}/{sb.pipelined}", flush=True)
+    assert sa.pipelined == 1 and sb.pipelined == 0
+    assert sa.done and sa.nsteps > 300
+    _same_state(sa, sb)
+    _same_model(a, b, a.grid)
+
+
+@pytest.mark.gpu
+def test_pipelined_run_read_mid_run_is_the_two_kernel_state():
+    """run_steps returns with no update waiting: a read between chunks (fields, maxima, tally, clock) sees the state
+    the two-kernel run has after the same steps."""
+    for n in (1, 2, 3, 17, 64, 200):
+        a, sa = _valley_run("hip-exact", True, 1, 2400.0, [(n, 0 if n < 16 else 16)])
+        b, sb = _valley_run("hip-exact", False, 1, 2400.0, [(n, 0 if n < 16 else 16)])
+        _same_state(sa, sb)
+        _same_model(a, b, a.grid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", ["hip-exact", "hip-f64"])
+def test_pipelined_ensemble_with_a_held_member_is_the_two_kernel_one(backend):
+    """Two members with SCS storms in one mosaic; one is held at t = 600 s for an odd number of the other's steps,
+    then both run to their ends."""
+    specs = [MemberSpec(0, 21, 96, 90.0, 0.5, 0.9), MemberSpec(1, 22, 64, 80.0, 0.5, 1.0)]
+    ens = Ensemble.build(specs, dx=20.0, k=4, duration_s=1500.0)
+    out = []
+    for pipeline in (True, False):
+        bt = setup_batched(ens, backend, maxima_every=4, pipeline=pipeline)
+        bt.model.backend.run_pause([600.0, np.inf])
+        s = bt.run.steps(40, 16)
+        while s[0].t < 600.0:
+            s = bt.run.steps(7, 0)
+        for chunk, graph in ((7, 0), (32, 16), (6, 0)):
+            s = bt.run.steps(chunk, graph)
+        bt.model.backend.run_pause([np.inf, np.inf])
+        s = bt.run.run(chunk=64, graph_steps=16)
+        out.append((bt, s))
+    (a, sa), (b, sb) = out
+    assert [x.pipelined for x in sa] == [1, 1] and [x.pipelined for x in sb] == [0, 0]
+    for m in range(2):
+        _same_state(sa[m], sb[m])
+    _same_model(a.model, b.model, ens.tables.grid)

@@ -254,11 +254,13 @@ __device__ __forceinline__ R cell_plan(R hb, R Hc, const float2* __restrict__ ta
 // the cell kernel reads the ring of a neighbour tile that another workgroup updates in the same launch, and the face
 // kernel writes the face velocity (and momentum) the next step reads while this step's are still read. The face
 // kernel reads level, discharge and wet share before the cell kernel rewrites them, so those are updated in place.
-// `qx1, qy1` hold the face discharges before the positivity limit, between the two kernels.
+// `qx1, qy1` hold the face discharges before the positivity limit, between the two kernels. The residual and the
+// discharges before the limit have two buffers as well: the pipelined step recomputes a neighbour tile's ring from its
+// old residual and discharges while that tile writes its new ones.
 template <class R>
 struct Bufs {
-    R *h[2], *u[2], *v[2], *mx[2], *my[2];
-    R *d, *qx, *qy, *om, *qx1, *qy1, *res, *src;
+    R *h[2], *u[2], *v[2], *mx[2], *my[2], *res[2], *qx1[2], *qy1[2];
+    R *d, *qx, *qy, *om, *src;
 };
 
 // A value every lane of the wave holds alike, kept in a scalar register (the compiler cannot see that a value loaded
@@ -270,7 +272,7 @@ __device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlan
 // momentum before the cell kernel overwrites it).
 template <class R>
 struct Cur {
-    R *h, *h_next, *u, *v, *u1, *v1, *mx, *my, *d, *qx, *qy, *om, *qx1, *qy1, *res, *src;
+    R *h, *h_next, *u, *v, *u1, *v1, *mx, *my, *d, *qx, *qy, *om, *qx1, *qy1, *res, *res_next, *src;
 };
 
 template <class R>
@@ -291,9 +293,10 @@ __host__ __device__ __forceinline__ Cur<R> view(const Bufs<R>& b, int par) {
     c.qx = b.qx;
     c.qy = b.qy;
     c.om = b.om;
-    c.qx1 = b.qx1;
-    c.qy1 = b.qy1;
-    c.res = b.res;
+    c.qx1 = p ? b.qx1[1] : b.qx1[0];  // written by this step's faces, read by its update
+    c.qy1 = p ? b.qy1[1] : b.qy1[0];
+    c.res = p ? b.res[1] : b.res[0];
+    c.res_next = p ? b.res[0] : b.res[1];
     c.src = b.src;
     return c;
 }
@@ -402,37 +405,50 @@ __device__ __forceinline__ Stager<NTH> make_stager(int t, const int* nb) {
     return s;
 }
 
-// W consecutive values, global -> registers -> LDS, as 16-byte (or 8-byte for two floats) accesses.
+// W consecutive values, global -> registers -> LDS, as 16-byte (or 8-byte for two floats) accesses; one value for a
+// workgroup of 1024 threads.
 template <int W>
 __device__ __forceinline__ void ldw(const float* __restrict__ g, float* o) {
     if constexpr (W == 4) {
         const float4 v = *reinterpret_cast<const float4*>(g);
         o[0] = v.x, o[1] = v.y, o[2] = v.z, o[3] = v.w;
-    } else {
-        static_assert(W == 2);
+    } else if constexpr (W == 2) {
         const float2 v = *reinterpret_cast<const float2*>(g);
         o[0] = v.x, o[1] = v.y;
+    } else {
+        static_assert(W == 1);
+        o[0] = *g;
     }
 }
 template <int W>
 __device__ __forceinline__ void ldw(const double* __restrict__ g, double* o) {
+    if constexpr (W == 1) {
+        o[0] = *g;
+    } else {
 #pragma unroll
-    for (int k = 0; k < W; k += 2) {
-        const double2 a = *reinterpret_cast<const double2*>(g + k);
-        o[k] = a.x, o[k + 1] = a.y;
+        for (int k = 0; k < W; k += 2) {
+            const double2 a = *reinterpret_cast<const double2*>(g + k);
+            o[k] = a.x, o[k + 1] = a.y;
+        }
     }
 }
 template <int W>
 __device__ __forceinline__ void stw(float* l, const float* o) {
     if constexpr (W == 4)
         *reinterpret_cast<float4*>(l) = make_float4(o[0], o[1], o[2], o[3]);
-    else
+    else if constexpr (W == 2)
         *reinterpret_cast<float2*>(l) = make_float2(o[0], o[1]);
+    else
+        *l = o[0];
 }
 template <int W>
 __device__ __forceinline__ void stw(double* l, const double* o) {
+    if constexpr (W == 1) {
+        *l = o[0];
+    } else {
 #pragma unroll
-    for (int k = 0; k < W; k += 2) *reinterpret_cast<double2*>(l + k) = make_double2(o[k], o[k + 1]);
+        for (int k = 0; k < W; k += 2) *reinterpret_cast<double2*>(l + k) = make_double2(o[k], o[k + 1]);
+    }
 }
 
 // N arrays into their LDS blocks: every load issued before the first store.
@@ -675,12 +691,20 @@ struct Member {
     int nlist;  // the member's active tiles, kept at seg[tile0 .. tile0 + nlist)
     int par;  // the member's buffer parity (see Cur): a member that does not step keeps its own
     double q_peak, t_peak;  // the largest discharge out through the member's tally faces after a step, and when
+    // The pipelined step (pipe_kernel): with `pipe`, a step whose next step keeps the list and reads no wet flag
+    // (`fuse`) leaves its cell update to the next step's pipe kernel; `pend` says an update waits there, with its
+    // step's parameters (`u_*`: length, wave step, new window and scale, parity, steps before it, clock after it).
+    int pipe, fuse, pend;
+    int u_want, u_new_window, u_par;
+    long long u_nsteps;
+    double u_dt, u_scale, u_time;
 };
 
 struct RunGlobals {
     int nact;
     int changed;   // a member's list changed since the active list was last gathered
     int pre_done;  // workgroups of the pre kernel that have finished (the last gathers the lists, then clears it)
+    int classic;   // a member's update this step is the cell kernel's (set by the pre kernel, cleared by the post kernel)
 };
 
 struct RunNum {
@@ -725,6 +749,97 @@ struct StepArgs {
     const int* bnd_tile;
 };
 
+// Both faces of own cell c (LDS entry L) of a tile: the x (west) face, then the y (north) face (faces.cu:244-271),
+// from the cell's momenta and the beds and wet shares of the face's two cells (west or north, own); writes the next
+// face velocities and the discharges before the limit of the view `s`, returns the larger wave speed.
+template <class R, bool FAST, bool LEARNED>
+__device__ __forceinline__ R face_cell(const StepArgs<R>& A, const Cur<R>& s, const FaceIn<R>& fx,
+                                       const FaceIn<R>& fy, int L, unsigned c, R mxo, R myo, R zxw, R zxo, R zyn,
+                                       R zyo, R omw, R omn, R omo, R dt, bool want_wave) {
+    const float2 e = ld_nt(A.st.es + c);
+    const R efx = R(fabsf(e.x)), efy = R(fabsf(e.y));
+    const R cwx = signbit(e.x) ? R(A.st.cwv[c].x) : R(0);
+    const R cwy = signbit(e.y) ? R(A.st.cwv[c].y) : R(0);
+    R un, mn, qn;
+    float2 cx{1.f, 0.f}, cy{1.f, 0.f};
+    if constexpr (LEARNED) {
+        cx = A.corr_x[c];
+        cy = A.corr_y[c];
+    }
+    const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), efx, cwx, zxw, zxo, A.st.ftabx + c,
+                                        A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c, omw,
+                                        omo, dt, want_wave, A.P, un, mn, qn, cx);
+    st_nt(un, s.u1 + c);
+    st_nt(qn, s.qx1 + c);
+    const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), efy, cwy, zyn, zyo, A.st.ftaby + c,
+                                        A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c, omn,
+                                        omo, dt, want_wave, A.P, un, mn, qn, cy);
+    st_nt(un, s.v1 + c);
+    st_nt(qn, s.qy1 + c);
+    return wx > wy ? wx : wy;
+}
+
+// The tile's largest wave speed, kept per tile and raised into its member's (a wave step).
+template <class R, int NTH>
+__device__ __forceinline__ void tile_wave(const StepArgs<R>& A, int t, int member, R w, R* red) {
+    const R m = block_max<NTH>(w, red);
+    if (threadIdx.x == 0) {
+        A.wave[t] = m;
+        if (A.member_wave) atomicMax(A.member_wave + member, wave_bits(m));
+    }
+}
+
+// The face step of tile t at parity `par` on the stored state: its arrays with the 2-cell ring into LDS (nb: the 3 x 3
+// block of tiles around t, loaded), then both own faces of every own cell.
+template <class R, bool FAST, int NTH, bool LEARNED>
+__device__ __forceinline__ void faces_tile(const StepArgs<R>& A, int t, int member, int par, R dt, bool want_wave,
+                                           const int* nb, R* s_u, R* s_v, R* s_h, R* s_d, R* s_qx, R* s_qy, R* red) {
+    const Cur<R> s = view(A.b, par);
+    __syncthreads();
+    {
+        const Stager<NTH> sg = make_stager<NTH>(t, nb);
+        R* const dst[6] = {s_u, s_v, s_h, s_d, s_qx, s_qy};
+        const R* const src[6] = {s.u, s.v, s.h, s.d, s.qx, s.qy};
+        stage<R, 6, NTH>(dst, src, sg);
+    }
+    __syncthreads();
+    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, LS};
+    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1};
+    const int i = threadIdx.x & (TILE - 1);
+    const bool fresh = A.fresh != 0;
+    const R dxr = R(A.P.dx);
+    R w = R(0);
+#pragma unroll
+    for (int r = 0; r < TILE2 / NTH; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+        const int L = lds_of(j, i);
+        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+        // the face momenta the last step left: Abar * u with Abar = 0.5 (hL + hR) dx from the depths it read, the
+        // same operations on the same operands as its own product, so the same bits as a stored copy
+        R mxo, myo;
+        if (fresh) {
+            mxo = s.mx[c];
+            myo = s.my[c];
+        } else {
+            const R ho = ld_nt(s.h_next + c);
+            mxo = R(0.5) * (s.h_next[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
+            myo = R(0.5) * (s.h_next[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
+        }
+        const unsigned kw = west_of(t, j, i, nb), kn = north_of(t, j, i, nb);
+        const R zxw = R(A.st.zcx[kw]), zxo = R(A.st.zcx[c]), zyn = R(A.st.zcy[kn]), zyo = R(A.st.zcy[c]);
+        R omw = R(0), omn = R(0), omo = R(0);
+        if (want_wave) {
+            omw = s.om[kw];
+            omn = s.om[kn];
+            omo = s.om[c];
+        }
+        const R m = face_cell<R, FAST, LEARNED>(A, s, fx, fy, L, c, mxo, myo, zxw, zxo, zyn, zyo, omw, omn, omo, dt,
+                                                want_wave);
+        w = m > w ? m : w;
+    }
+    if (want_wave) tile_wave<R, NTH>(A, t, member, w, red);
+}
+
 // Both own faces of every own cell of one active tile per workgroup of NTH threads (faces.cu:244-271).
 template <class R, bool FAST, int NTH, bool LEARNED = false>
 __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
@@ -750,78 +865,9 @@ __global__ __launch_bounds__(NTH) void faces_kernel(StepArgs<R> A) {
         want_run = uni(mb.refresh);
         par = uni(mb.par);
     }
-    const Cur<R> s = view(A.b, par);
     load_nb3(t, A.st.nbr, nb);
-    __syncthreads();
-    {
-        const Stager<NTH> sg = make_stager<NTH>(t, nb);
-        R* const dst[6] = {s_u, s_v, s_h, s_d, s_qx, s_qy};
-        const R* const src[6] = {s.u, s.v, s.h, s.d, s.qx, s.qy};
-        stage<R, 6, NTH>(dst, src, sg);
-    }
-    __syncthreads();
-    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, LS};
-    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1};
-    const int i = threadIdx.x & (TILE - 1);
-    const bool want_wave = want_run != 0;
-    const bool fresh = A.fresh != 0;
-    const R dxr = R(A.P.dx);
-    const R dt = dt_run;
-    R w = R(0);
-#pragma unroll
-    for (int r = 0; r < TILE2 / NTH; ++r) {
-        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
-        const int L = lds_of(j, i);
-        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
-        const float2 e = ld_nt(A.st.es + c);
-        const R efx = R(fabsf(e.x)), efy = R(fabsf(e.y));
-        const R cwx = signbit(e.x) ? R(A.st.cwv[c].x) : R(0);
-        const R cwy = signbit(e.y) ? R(A.st.cwv[c].y) : R(0);
-        R un, mn, qn;
-        float2 cx{1.f, 0.f}, cy{1.f, 0.f};
-        if constexpr (LEARNED) {
-            cx = A.corr_x[c];
-            cy = A.corr_y[c];
-        }
-        // the face momenta the last step left: Abar * u with Abar = 0.5 (hL + hR) dx from the depths it read, the
-        // same operations on the same operands as its own product, so the same bits as a stored copy
-        R mxo, myo;
-        if (fresh) {
-            mxo = s.mx[c];
-            myo = s.my[c];
-        } else {
-            const R ho = ld_nt(s.h_next + c);
-            mxo = R(0.5) * (s.h_next[west_of(t, j, i, nb)] + ho) * dxr * s_u[L];
-            myo = R(0.5) * (s.h_next[north_of(t, j, i, nb)] + ho) * dxr * s_v[L];
-        }
-        const unsigned kw = west_of(t, j, i, nb), kn = north_of(t, j, i, nb);
-        const R zxw = R(A.st.zcx[kw]), zxo = R(A.st.zcx[c]), zyn = R(A.st.zcy[kn]), zyo = R(A.st.zcy[c]);
-        R omw = R(0), omn = R(0), omo = R(0);
-        if (want_wave) {
-            omw = s.om[kw];
-            omn = s.om[kn];
-            omo = s.om[c];
-        }
-        const R wx = face<R, FAST, LEARNED>(fx, L, mxo, ld_nt(A.st.sx + c), efx, cwx, zxw, zxo, A.st.ftabx + c,
-                                            A.st.ncell, A.st.last, A.st.fcx ? A.st.fcx + c : nullptr, A.st.ftagx + c,
-                                            omw, omo, dt, want_wave, A.P, un, mn, qn, cx);
-        st_nt(un, s.u1 + c);
-        st_nt(qn, s.qx1 + c);
-        const R wy = face<R, FAST, LEARNED>(fy, L, myo, ld_nt(A.st.sy + c), efy, cwy, zyn, zyo, A.st.ftaby + c,
-                                            A.st.ncell, A.st.last, A.st.fcy ? A.st.fcy + c : nullptr, A.st.ftagy + c,
-                                            omn, omo, dt, want_wave, A.P, un, mn, qn, cy);
-        st_nt(un, s.v1 + c);
-        st_nt(qn, s.qy1 + c);
-        const R m = wx > wy ? wx : wy;
-        w = m > w ? m : w;
-    }
-    if (want_wave) {
-        const R m = block_max<NTH>(w, red);
-        if (threadIdx.x == 0) {
-            A.wave[t] = m;
-            if (A.member_wave) atomicMax(A.member_wave + member, wave_bits(m));
-        }
-    }
+    faces_tile<R, FAST, NTH, LEARNED>(A, t, member, par, dt_run, want_run != 0, nb, s_u, s_v, s_h, s_d, s_qx, s_qy,
+                                      red);
 }
 
 // The learned closure on its faces, before a step that refreshes the wave speed (every dt_every steps and after a
@@ -1024,7 +1070,7 @@ __device__ __forceinline__ bool update_tile(const StepArgs<R>& A, const Cur<R>& 
             s.src[c] = src_v[r];
             src_sum += (double)src_v[r];
         }
-        st_nt(resn_v[r], s.res + c);
+        st_nt(resn_v[r], s.res_next + c);
         const R hn = hn_v[r];
         st_nt(hn, s.h_next + c);
         const R Hc = R(hc_v[r]);
@@ -1120,7 +1166,7 @@ __device__ __forceinline__ bool tile_step(const StepArgs<R>& A, int t, TileStep<
     if (A.members) {
         ts.member = uni(A.tile_member[t]);
         const Member& mb = A.members[ts.member];
-        if (!uni(mb.stepping)) return false;
+        if (!uni(mb.stepping) || uni(mb.fuse)) return false;  // a waiting update is the next pipe kernel's
         ts.dt = (R)mb.dt_step;
         ts.want = uni(mb.refresh);
         ts.new_window = uni(mb.new_window) && !A.excess;
@@ -1139,6 +1185,7 @@ __device__ __forceinline__ bool tile_step(const StepArgs<R>& A, int t, TileStep<
 // residual, the level and wet share of every own cell (cells.cu:28-63): the second kernel of the two-pass step.
 template <class R, bool FAST, int NTH>
 __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
+    if (A.members && !A.glob->classic) return;  // every member's update waits for the next pipe kernel
     __shared__ __align__(16) R s_h[LSIZE];
     __shared__ __align__(16) R s_qx[LSIZE];
     __shared__ __align__(16) R s_qy[LSIZE];
@@ -1169,6 +1216,407 @@ __global__ __launch_bounds__(NTH) void cells_kernel(StepArgs<R> A) {
     const bool wet = update_tile<R, FAST, NTH, 2>(A, s, t, s_h, s_qx, s_qy, s_r, ts.dt, ts.want != 0,
                                                   ts.new_window != 0, ts.scale, src_sum, ts.sample != 0, ts.time, nb);
     tile_flags<NTH>(t, wet, ts.new_window != 0, src_sum, &any_wet, red_src, A.wet, A.src_partial);
+}
+
+// ------------------------------------------------------------------------------------------------------------
+// the pipelined step
+
+// What the pipe kernel reads beyond a step's arguments: which tiles are listed (a neighbour tile not listed, or the
+// void tile, keeps its stored state), and for the tiles that need them each cell's boundary entry and tally faces.
+struct PipeArgs {
+    const int* is_active;  // per tile, the device loop's list as flags
+    const int* near;       // per tile: bit 0 a boundary cell in the tile or its west or north neighbour, bit 1 a tally
+                           // face in the tile
+    const int* bidx;       // per cell: its entry in the boundary lists, -1 for none
+    const int2* tidx;      // per cell: its x and y tally faces, -1 for none
+    const int* b_kind;
+    const int* b_ref;
+    const float* b_d0;
+    const float* b_h0;
+    const float* t_sign;
+    double* t_total;
+    double* t_flow;
+    int drain;  // the waiting updates only: no face step
+};
+
+// Which of the 3 x 3 tiles around a tile holds its cell (j, i), j and i in [-32, 63], and the cell's state index.
+__device__ __forceinline__ int nb_of(int j, int i) {
+    return (j < 0 ? 0 : (j >= TILE ? 2 : 1)) * 3 + (i < 0 ? 0 : (i >= TILE ? 2 : 1));
+}
+__device__ __forceinline__ unsigned cell_of(const int* nb, int j, int i) {
+    const int q = nb_of(j, i);
+    const int dj = q / 3 - 1, di = q - (q / 3) * 3 - 1;
+    return (unsigned)(nb[q] * TILE2 + (j - dj * TILE) * TILE + (i - di * TILE));
+}
+
+// One step of a member whose last cell update waits (Member::pend), in one pass over each tile: that update, then this
+// step's faces, which read its results from LDS instead of from memory. The update's inputs (the mean depth, residual
+// and the discharges before the limit of the step before, with the 2-cell ring) come from the buffers of its own
+// parity, which no workgroup writes in this launch; each tile recomputes the update of its west column and north row
+// neighbours (whose faces and levels its own faces read) from the same inputs with the same statements as their own
+// tile does, and takes the stored state of a neighbour tile that is not listed. A member with nothing waiting takes
+// the face step of faces_kernel. With `drain`, only the waiting updates are made.
+//   1. positivity ratio on the tile and a 2-cell ring (cells.cu:4-18), from the staged depth and discharges;
+//   2. the face momenta of the own faces (Abar * u from the depths the last face step read: the staged ones);
+//   3. the discharges limited by the upwind cell's ratio, in place (cells.cu:36-43);
+//   4. continuity, rain, residual, level and wet share of the own cells and of the west column and north row
+//      (cells.cu:44-63), stored for the own cells, into LDS for all;
+//   5. boundary cells among them (cells.cu:95-111; a follower's reference cell is one of them, checked when the run
+//      is set up);
+//   6. the own cells' running maxima, boundary tally (engine/maxima.py:29-62), wet flag and window source;
+//   7. stored discharges of faces whose cell is in a tile that is not listed, as faces_kernel would stage them;
+//   8. both faces of every own cell (faces.cu:244-271).
+template <class R, bool FAST, int NTH>
+__global__ __launch_bounds__(NTH, 6) void pipe_kernel(StepArgs<R> A, PipeArgs G) {
+    __shared__ __align__(16) R s_u[LSIZE];
+    __shared__ __align__(16) R s_v[LSIZE];
+    __shared__ __align__(16) R s_h[LSIZE];  // the old mean depth, then the new one
+    __shared__ __align__(16) R s_d[LSIZE];
+    __shared__ __align__(16) R s_qx[LSIZE];  // the discharges before the limit, then after it
+    __shared__ __align__(16) R s_qy[LSIZE];
+    __shared__ __align__(16) R s_r[LSIZE];  // the positivity ratio, then the wet share
+    __shared__ int nb[9], live[9];
+    __shared__ R red[NTH / 64];
+    __shared__ int any_wet;
+    __shared__ double red_src[NTH / 64];
+    constexpr int RO = TILE2 / NTH;
+    const int slot = xcd_slot(blockIdx.x, A.glob->nact);
+    if (slot < 0) return;
+    const int t = uni(A.active[slot]);
+    const int member = uni(A.tile_member[t]);
+    const Member& mb = A.members[member];
+    const bool pend = uni(mb.pend) != 0;
+    const bool step = !G.drain && uni(mb.stepping) != 0;
+    if (!pend && !step) return;
+    load_nb3(t, A.st.nbr, nb);
+    if (!pend) {
+        faces_tile<R, FAST, NTH, false>(A, t, member, uni(mb.par), (R)mb.dt_step, uni(mb.refresh) != 0, nb, s_u, s_v,
+                                        s_h, s_d, s_qx, s_qy, red);
+        return;
+    }
+    const int up = uni(mb.u_par);
+    const Cur<R> su = view(A.b, up);      // the waiting update
+    const Cur<R> sf = view(A.b, up ^ 1);  // this step's faces
+    const R dt_u = (R)mb.u_dt;
+    const bool plan_u = uni(mb.u_want) != 0;
+    const bool win_u = uni(mb.u_new_window) != 0 && !A.excess;
+    const double scale_u = mb.u_scale;
+    const bool sample = A.every > 0 && (mb.u_nsteps + 1) % A.every == 0;
+    const bool want_wave = step && uni(mb.refresh) != 0;
+    const int flags = uni(G.near[t]);
+    const bool failed = uni(mb.failed) != 0;  // the waiting step failed: its tally is not booked
+    // the level and limited discharges of the update are stored only when no face step follows in this launch (a
+    // drain, a held or failed member): a following step's update, pipelined or not, rewrites them before anything
+    // reads them, and a tile leaves the list only after a step whose update the cell kernel made
+    const bool keep = !step;
+    if (threadIdx.x < 9) live[threadIdx.x] = nb[threadIdx.x] != 0 && G.is_active[nb[threadIdx.x]] != 0;
+    if (threadIdx.x == 0) any_wet = 0;
+    __syncthreads();
+    {
+        const Stager<NTH> sg = make_stager<NTH>(t, nb);
+        R* const dst[5] = {s_h, s_qx, s_qy, s_u, s_v};
+        const R* const src[5] = {su.h, su.qx1, su.qy1, sf.u, sf.v};
+        stage<R, 5, NTH>(dst, src, sg);
+    }
+    __syncthreads();
+    const R dx = R(A.P.dx);
+    const R area = dx * dx;
+    const int i = threadIdx.x & (TILE - 1);
+    // the update's inputs from memory, loaded now so that they arrive during phases 1-3: the own cells' residual,
+    // source and table top (ordinary loads: the east and south neighbours read the edge ones again) and row tag ...
+    unsigned cs[RO];
+    R res_v[RO], src_v[RO];
+    float hc_v[RO];
+    int tag_v[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+        cs[r] = c;
+        res_v[r] = su.res[c];
+        hc_v[r] = A.st.hc[c];
+        tag_v[r] = ld_nt(A.st.ctag + c);
+        if (win_u)  // the window's rate of this cell: float(weight * scale), the product in double
+            src_v[r] = A.src_weight ? (R)(A.src_weight[c] * scale_u) : R(0);
+        else
+            src_v[r] = su.src[c];
+    }
+    // ... and those of the west column and north row, RPW cells in the first lanes of every wave
+    constexpr int RPW = 2 * TILE / (NTH / 64);
+    const int lane = threadIdx.x & 63;
+    const int rq = lane < RPW ? (int)(threadIdx.x >> 6) * RPW + lane : -1;
+    const bool r_west = rq >= 0 && rq < TILE;
+    const int r_j = rq < 0 ? 0 : (r_west ? rq : -1), r_i = rq < 0 ? 0 : (r_west ? -1 : rq - TILE);
+    const int r_q = r_west ? 3 : 1;
+    const unsigned r_c = rq < 0 ? 0u : cell_of(nb, r_j, r_i);
+    const bool r_live = rq >= 0 && live[r_q];
+    R r_res = R(0), r_src = R(0);
+    float r_hc = 0.f;
+    if (r_live) {
+        r_res = su.res[r_c];
+        r_hc = A.st.hc[r_c];
+        r_src = win_u ? (A.src_weight ? (R)(A.src_weight[r_c] * scale_u) : R(0)) : su.src[r_c];
+    }
+    // 1. positivity ratio on cells [-2, 32]^2
+    for (int p = threadIdx.x; p < 35 * 35; p += NTH) {
+        const int jj = p / 35 - 2, ii = p - (p / 35) * 35 - 2;
+        const int L = lds_of(jj, ii);
+        const R out = pmax(s_qx[L + 1], R(0)) + pmax(-s_qx[L], R(0)) + pmax(s_qy[L + LS], R(0)) +
+                      pmax(-s_qy[L], R(0));
+        const R have = pmax(s_h[L], R(0)) * area;
+        s_r[L] = out * dt_u > have ? dv<FAST>(have, out * dt_u) : R(1);
+    }
+    // 2. the own faces' momenta from the depths the last face step read
+    R mxo[RO], myo[RO];
+#pragma unroll
+    for (int r = 0; r < RO; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+        const int L = lds_of(j, i);
+        mxo[r] = R(0.5) * (s_h[L - 1] + s_h[L]) * dx * s_u[L];
+        myo[r] = R(0.5) * (s_h[L - LS] + s_h[L]) * dx * s_v[L];
+    }
+    __syncthreads();
+    // 3. limited discharges: x faces of cells [-1, 31] x [-1, 32], y faces of cells [-1, 32] x [-1, 31]
+    for (int p = threadIdx.x; p < 33 * 34; p += NTH) {
+        const int a = p / 34 - 1, b = p - (p / 34) * 34 - 1;
+        const int Lx = lds_of(a, b);
+        s_qx[Lx] = scaled(s_qx[Lx], s_r[Lx - 1], s_r[Lx]);
+        const int Ly = lds_of(b, a);
+        s_qy[Ly] = scaled(s_qy[Ly], s_r[Ly - LS], s_r[Ly]);
+    }
+    __syncthreads();
+    // 4. the update of the own cells (three passes, as update_tile) ...
+    bool wet = false;
+    double src_sum = 0.0;
+    {
+        R hn_v[RO], resn_v[RO], s_v2[RO];
+        int row_v[RO], mode_v[RO];
+        float2 pair_v[RO];
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+            const int L = lds_of(j, i);
+            const unsigned c = cs[r];
+            const R qw = s_qx[L], qe = s_qx[L + 1], qn = s_qy[L], qs = s_qy[L + LS];
+            const R h0 = s_h[L];
+            const R inc = res_v[r] + dt_u * src_v[r] - dv<FAST>(dt_u * (qe - qw + qs - qn), area);
+            const R sum = h0 + inc;
+            const R hn = pmax(sum, R(0));
+            resn_v[r] = sum > R(0) ? inc - (sum - h0) : R(0);
+            hn_v[r] = hn;
+            const R Hc = R(hc_v[r]);
+            int mode = 0, row = 0;
+            R sv = R(0);
+            if (Hc > R(0)) {
+                if (hn >= Hc) {
+                    mode = 1;
+                    row = A.st.last - 1;
+                } else {
+                    mode = 2;
+                    sv = sq<FAST>(dv<FAST>(hn, Hc)) * R(A.st.last);
+                    row = row_of(sv, A.st.last - 1);
+                }
+            }
+            mode_v[r] = mode;
+            row_v[r] = row;
+            s_v2[r] = sv;
+            if (mode) pair_v[r] = tag_v[r] == row ? A.st.cc[c] : A.st.ctab[(size_t)row * A.st.ncell + c];
+        }
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+            const int L = lds_of(j, i);
+            const unsigned c = cs[r];
+            if (keep) {
+                st_nt(s_qx[L], su.qx + c);
+                st_nt(s_qy[L], su.qy + c);
+            }
+            if (win_u) {
+                su.src[c] = src_v[r];
+                src_sum += (double)src_v[r];
+            }
+            st_nt(resn_v[r], su.res_next + c);
+            const R hn = hn_v[r];
+            st_nt(hn, su.h_next + c);
+            const R Hc = R(hc_v[r]);
+            const float2 q = pair_v[r];
+            R dn, plan = R(1);
+            if (mode_v[r] == 0) {
+                dn = hn;
+            } else if (mode_v[r] == 1) {
+                dn = R(q.y) + (hn - Hc);
+            } else {
+                const int row = row_v[r];
+                if (plan_u) {
+                    const R m = R(A.st.last);
+                    plan = dv<FAST>(dv<FAST>(Hc * R(2 * row + 1), m * m), pmax(R(q.y) - R(q.x), R(1e-12)));
+                }
+                dn = R(q.x) + (s_v2[r] - R(row)) * (R(q.y) - R(q.x));
+            }
+            if (mode_v[r] && tag_v[r] != row_v[r]) {  // refill the cell's row cache
+                A.st.cc[c] = q;
+                A.st.ctag[c] = (unsigned char)row_v[r];
+            }
+            if (keep) st_nt(dn, su.d + c);
+            if (plan_u) st_nt(plan, su.om + c);
+            s_h[L] = hn;
+            s_d[L] = dn;
+            if (want_wave) s_r[L] = plan_u ? plan : su.om[c];
+            wet |= hn > R(0);
+        }
+    }
+    // ... and of the west column and north row
+    if (rq >= 0) {
+        const unsigned c = r_c;
+        const int L = lds_of(r_j, r_i);
+        if (r_live) {
+            const R Hc = R(r_hc);
+            const R qw = s_qx[L], qe = s_qx[L + 1], qn = s_qy[L], qs = s_qy[L + LS];
+            const R h0 = s_h[L];
+            const R inc = r_res + dt_u * r_src - dv<FAST>(dt_u * (qe - qw + qs - qn), area);
+            const R sum = h0 + inc;
+            const R hn = pmax(sum, R(0));
+            const float2* tab = A.st.ctab + c;
+            s_h[L] = hn;
+            s_d[L] = cell_level<R, FAST>(hn, Hc, tab, A.st.ncell, A.st.last);
+            if (want_wave) s_r[L] = plan_u ? cell_plan<R, FAST>(hn, Hc, tab, A.st.ncell, A.st.last) : su.om[c];
+        } else {  // the stored state (its depth is in both buffers, so the staged one)
+            s_d[L] = su.d[c];
+            if (want_wave) s_r[L] = su.om[c];
+        }
+    }
+    __syncthreads();
+    // 5. boundary cells among the updated ones (their values after the step, as post_kernel sets them)
+    if (flags & 1) {
+        constexpr int NC = RO + 1;
+        R hb[NC], db[NC];
+        int Lb[NC];
+        unsigned cb[NC];
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            Lb[r] = -1;
+            int jj, ii, q;
+            if (r < RO) {
+                jj = (threadIdx.x >> 5) + r * (NTH / TILE);
+                ii = i;
+                q = 4;
+            } else {
+                if (rq < 0) continue;
+                jj = r_j;
+                ii = r_i;
+                q = r_q;
+            }
+            if (!live[q]) continue;
+            const unsigned c = cell_of(nb, jj, ii);
+            const int e = G.bidx[c];
+            if (e < 0) continue;
+            const int kind = G.b_kind[e];
+            if (kind == 0) {
+                hb[r] = R(G.b_h0[e]);
+                db[r] = R(G.b_d0[e]);
+            } else if (kind == 1) {
+                hb[r] = R(0);
+                db[r] = R(0);
+            } else {
+                const int rf = G.b_ref[e], rt = rf / TILE2, rl = rf - rt * TILE2;
+                int qr = -1;
+                for (int k = 0; k < 9; ++k)
+                    if (qr < 0 && nb[k] == rt && rt != 0) qr = k;
+                if (qr < 0) continue;  // not among the updated cells: the run is set up so that this never happens
+                const int Lr = lds_of((rl >> 5) + (qr / 3 - 1) * TILE, (rl & (TILE - 1)) + (qr % 3 - 1) * TILE);
+                hb[r] = s_h[Lr];
+                db[r] = pmax(s_d[Lr] + R(G.b_d0[e]), R(0));
+            }
+            Lb[r] = lds_of(jj, ii);
+            cb[r] = c;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < NC; ++r) {
+            if (Lb[r] < 0) continue;
+            s_h[Lb[r]] = hb[r];
+            s_d[Lb[r]] = db[r];
+            if (r < RO) {
+                su.h_next[cb[r]] = hb[r];
+                if (keep) su.d[cb[r]] = db[r];
+            }
+        }
+        __syncthreads();
+    }
+    // 6. running maxima, tally, wet flag and window source of the own cells
+    if (sample) {
+        const R time = (R)mb.u_time;
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+            const int L = lds_of(j, i);
+            const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+            const R dk = s_d[L], hk = s_h[L];
+            if (dk > A.mx_d[c]) A.mx_d[c] = dk;
+            if (hk > A.mx_h[c]) A.mx_h[c] = hk;
+            if (dk > A.deep) {
+                const R uc = R(0.5) * (s_u[L] + s_u[L + 1]);
+                const R vc = R(0.5) * (s_v[L] + s_v[L + LS]);
+                const R sp = sqrt(uc * uc + vc * vc);
+                if (sp > A.mx_s[c]) A.mx_s[c] = sp;
+                if (sp * dk > A.mx_z[c]) A.mx_z[c] = sp * dk;
+            }
+            if (A.mx_t[c] < R(0) && dk >= A.arrive) A.mx_t[c] = time;
+        }
+    }
+    if (flags & 2) {
+#pragma unroll
+        for (int r = 0; r < RO; ++r) {
+            const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+            const int L = lds_of(j, i);
+            const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+            const int2 tq = G.tidx[c];
+            if (tq.x >= 0) {
+                const double f = (double)(R(G.t_sign[tq.x]) * s_qx[L]);
+                if (!failed) G.t_total[tq.x] += mb.u_dt * f;
+                G.t_flow[tq.x] = f;
+            }
+            if (tq.y >= 0) {
+                const double f = (double)(R(G.t_sign[tq.y]) * s_qy[L]);
+                if (!failed) G.t_total[tq.y] += mb.u_dt * f;
+                G.t_flow[tq.y] = f;
+            }
+        }
+    }
+    tile_flags<NTH>(t, wet, win_u, src_sum, &any_wet, red_src, A.wet, A.src_partial);
+    if (!step) return;
+    // 7. a face whose cell lies in a tile that is not listed keeps its stored discharge
+    if (!(live[0] && live[1] && live[2] && live[3] && live[5] && live[6] && live[7] && live[8])) {
+        for (int p = threadIdx.x; p < 33 * 34; p += NTH) {
+            const int a = p / 34 - 1, b = p - (p / 34) * 34 - 1;
+            if (!live[nb_of(a, b)]) s_qx[lds_of(a, b)] = su.qx[cell_of(nb, a, b)];
+            if (!live[nb_of(b, a)]) s_qy[lds_of(b, a)] = su.qy[cell_of(nb, b, a)];
+        }
+        __syncthreads();
+    }
+    // 8. this step's faces
+    const FaceIn<R> fx{s_u, s_v, s_qx, s_qy, s_h, s_d, 1, LS};
+    const FaceIn<R> fy{s_v, s_u, s_qy, s_qx, s_h, s_d, LS, 1};
+    const R dt_f = (R)mb.dt_step;
+    R w = R(0);
+#pragma unroll 1  // one cell at a time: 6 waves per SIMD without spilling
+    for (int r = 0; r < RO; ++r) {
+        const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
+        const int L = lds_of(j, i);
+        const unsigned c = (unsigned)(t * TILE2 + j * TILE + i);
+        const unsigned kw = west_of(t, j, i, nb), kn = north_of(t, j, i, nb);
+        const R zxw = R(A.st.zcx[kw]), zxo = R(A.st.zcx[c]), zyn = R(A.st.zcy[kn]), zyo = R(A.st.zcy[c]);
+        R omw = R(0), omn = R(0), omo = R(0);
+        if (want_wave) {
+            omw = s_r[L - 1];
+            omn = s_r[L - LS];
+            omo = s_r[L];
+        }
+        const R m = face_cell<R, FAST, false>(A, sf, fx, fy, L, c, mxo[r], myo[r], zxw, zxo, zyn, zyo, omw, omn, omo,
+                                              dt_f, want_wave);
+        w = m > w ? m : w;
+    }
+    if (want_wave) tile_wave<R, NTH>(A, t, member, w, red);
 }
 
 // Wet share of the listed tiles' cells (cells.cu:66-77).
@@ -1204,13 +1652,17 @@ __global__ void boundary_kernel(int n, const int* __restrict__ cell, const int* 
     }
 }
 
-// Tiles that leave the active list (a protocol set_active) keep one mean depth in both buffers, at the handle's parity.
+// Tiles that leave the active list (a protocol set_active) keep one mean depth and one residual in both buffers, at
+// the handle's parity.
 template <class R>
 __global__ void leave_tiles_kernel(const int* __restrict__ tiles, int n, Bufs<R> B, int par) {
     if ((int)blockIdx.x >= n) return;
     const Cur<R> s = view(B, par);
     const size_t base = (size_t)tiles[blockIdx.x] * TILE2;
-    for (int p = threadIdx.x; p < TILE2; p += NT) s.h_next[base + p] = s.h[base + p];
+    for (int p = threadIdx.x; p < TILE2; p += NT) {
+        s.h_next[base + p] = s.h[base + p];
+        s.res_next[base + p] = s.res[base + p];
+    }
 }
 
 template <class R>
@@ -1233,6 +1685,7 @@ __global__ void align_kernel(int t0, int t1, Bufs<R> B, int par, int par2) {
         swap_at(B.v[0], B.v[1], c);
         swap_at(B.mx[0], B.mx[1], c);
         swap_at(B.my[0], B.my[1], c);
+        swap_at(B.res[0], B.res[1], c);
     }
 }
 
@@ -1444,11 +1897,13 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.new_window = 0;
     b.refresh = 0;
     b.rebuild = 0;
+    b.fuse = 0;
     if (b.done) return;
     if (b.t >= b.t_stop) return;  // held where the host asked (its end too): it reads this state before the next step
     // a failed member the host holds (a finite stop time: a job's chunks) stays listed as it failed, so the host
-    // reads the failing step's state and samples its maxima over the same tiles as the host loop would
-    if (b.failed && b.t_stop < INFINITY) return;
+    // reads the failing step's state and samples its maxima over the same tiles as the host loop would; a failed
+    // member whose last update waits for the pipe kernel is held for that step and retired after it
+    if (b.failed && (b.t_stop < INFINITY || b.pend)) return;
     if (b.failed || b.t >= b.t_end - 1e-9) {
         b.done = 1;
         b.rebuild = 3;
@@ -1492,6 +1947,14 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.refresh_due = 0;
     b.dt_step = fmin(fmin(b.dt, b.window_end - b.t), num.dt_max);
     b.stepping = 1;
+    // this step's update is left to the next step's pipe kernel unless the next step reads the wet flags it leaves (a
+    // check follows a step whose count is a multiple of active_every) or may change the list or the source (a window
+    // end, the run's end or a hold: the clock after this step is post_kernel's t + dt_step, the same sum)
+    if (b.pipe) {
+        const double t1 = b.t + b.dt_step;
+        b.fuse = (b.nsteps % num.active_every) != 0 && !(t1 >= b.window_end - 1e-9) && !(t1 >= b.t_end - 1e-9) &&
+                 !(t1 >= b.t_stop);
+    }
 }
 
 // Whether ctl_begin opens a window of this member before its next step (the clock has reached the window's end and
@@ -1799,21 +2262,24 @@ __device__ void transitions(const Member& b, int nw, int nl, const RunLists& L, 
         }
         __syncthreads();
     }
-    // then one mean depth in both buffers, and for a finished member one velocity (the momentum is in both already)
+    // then one mean depth and one residual in both buffers, and for a finished member one velocity (the momentum is in
+    // both already)
     for (int q0 = 0; q0 < nl; q0 += TB) {
-        R hv[TB], uv[TB], vv[TB];
+        R hv[TB], rv[TB], uv[TB], vv[TB];
         size_t cv[TB];
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
             if (q0 + u >= nl) continue;
             cv[u] = (size_t)L.leaving[b.tile0 + q0 + u] * TILE2 + p;
             hv[u] = s.h[cv[u]];
+            rv[u] = s.res[cv[u]];
             if (b.done) uv[u] = s.u[cv[u]], vv[u] = s.v[cv[u]];
         }
 #pragma unroll
         for (int u = 0; u < TB; ++u) {
             if (q0 + u >= nl) continue;
             s.h_next[cv[u]] = hv[u];
+            s.res_next[cv[u]] = rv[u];
             if (b.done) s.u1[cv[u]] = uv[u], s.v1[cv[u]] = vv[u];
         }
     }
@@ -1925,6 +2391,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
             int mine = 0;
             ctl_begin(b, scales, num, ex_sum, mine);
             if (b.refresh) member_wave[blockIdx.x] = 0ull;
+            if (b.stepping && !b.fuse) g->classic = 1;
             any = mine;
         }
         __syncthreads();
@@ -1970,65 +2437,107 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
                                                    const float* __restrict__ h0, Bufs<R> B,
                                                    const int* __restrict__ face, const int* __restrict__ axis,
                                                    const float* __restrict__ sign, double* __restrict__ total,
-                                                   double* __restrict__ flow) {
+                                                   double* __restrict__ flow, int drain, RunGlobals* g) {
     __shared__ double red_s[POST_THREADS / 64];
     __shared__ double red_q[POST_THREADS / 64];
+    if (blockIdx.x == 0 && threadIdx.x == 0) g->classic = 0;
     Member& b = mem[blockIdx.x];
-    if (!b.stepping) return;
+    const bool pend = b.pend != 0;
+    const bool step = !drain && b.stepping;
+    if (!pend && !step) return;
+    // the window's rain of a step that opened one: the source sums of the member's listed tiles
+    auto rain_sum = [&]() {
+        double rs = 0.0;
+        for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x)
+            if (is_active[t]) rs += src_partial[t];
+        for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off);
+        if ((threadIdx.x & 63) == 0) red_s[threadIdx.x >> 6] = rs;
+    };
+    auto reduced = [&](const double* red) {
+        double v = red[0];
+        for (int q = 1; q < POST_THREADS / 64; ++q) v += red[q];
+        return v;
+    };
+    if (pend) {
+        // the last step's update, just made by the pipe kernel with its boundary cells and tally: its peak outflow
+        // from the flows it left, and its window's rain
+        double out_q = 0.0;
+        for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) out_q += flow[q];
+        for (int off = 32; off > 0; off >>= 1) out_q += __shfl_xor(out_q, off);
+        if ((threadIdx.x & 63) == 0) red_q[threadIdx.x >> 6] = out_q;
+        const bool books_rain = b.u_new_window && !num.excess;
+        if (books_rain) rain_sum();
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (books_rain) b.rain_window = reduced(red_s) * num.area * (b.window_end - b.window_start);
+            if (b.f1 > b.f0 && !b.failed) {
+                const double q = reduced(red_q);
+                if (q > b.q_peak) {
+                    b.q_peak = q;
+                    b.t_peak = b.u_time;
+                }
+            }
+            b.pend = 0;
+        }
+        __syncthreads();
+        if (!step) return;
+    }
+    const bool fused = b.fuse != 0;  // this step's update waits for the next step's pipe kernel
     // the step's outputs: the next buffers of the parities it stepped at (thread 0 advances them below)
     const Cur<R> s = view(B, b.par);
     R* const h = s.h_next;
     R* const d = s.d;
-    for (int p = b.b0 + threadIdx.x; p < b.b1; p += blockDim.x) {
-        const int k = cell[p];
-        if (kind[p] == 0) {
-            h[k] = R(h0[p]);
-            d[k] = R(d0[p]);
-        } else if (kind[p] == 1) {
-            h[k] = R(0);
-            d[k] = R(0);
-        } else {
-            h[k] = h[ref[p]];
-            d[k] = pmax(d[ref[p]] + R(d0[p]), R(0));
+    if (!fused)
+        for (int p = b.b0 + threadIdx.x; p < b.b1; p += blockDim.x) {
+            const int k = cell[p];
+            if (kind[p] == 0) {
+                h[k] = R(h0[p]);
+                d[k] = R(d0[p]);
+            } else if (kind[p] == 1) {
+                h[k] = R(0);
+                d[k] = R(0);
+            } else {
+                h[k] = h[ref[p]];
+                d[k] = pmax(d[ref[p]] + R(d0[p]), R(0));
+            }
         }
-    }
     // the step's wave speed and whether it fails (stepper.py:117-124), known to every thread before the books: a
     // failing step books nothing, as the host loop raises before its after-step work (tally, smallest dt, peak)
     const double w = b.refresh ? wave_value<R>(member_wave[blockIdx.x]) : 0.0;
     const double dt_next = b.refresh ? fmin(num.cfl * num.dx / fmax(w, num.wave_floor), num.dt_max) : b.dt;
     const int failing = !b.refresh ? 0 : !isfinite(w) ? 1 : dt_next < num.dt_min ? 2 : 0;
     double out_q = 0.0;  // this thread's share of the member's discharge out, summed in a fixed order
-    for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) {
-        const R qq = axis[q] ? s.qy[face[q]] : s.qx[face[q]];
-        const double f = (double)(R(sign[q]) * qq);
-        if (!failing) total[q] += b.dt_step * f;
-        flow[q] = f;
-        out_q += f;
-    }
+    if (!fused)
+        for (int q = b.f0 + threadIdx.x; q < b.f1; q += blockDim.x) {
+            const R qq = axis[q] ? s.qy[face[q]] : s.qx[face[q]];
+            const double f = (double)(R(sign[q]) * qq);
+            if (!failing) total[q] += b.dt_step * f;
+            flow[q] = f;
+            out_q += f;
+        }
     for (int off = 32; off > 0; off >>= 1) out_q += __shfl_xor(out_q, off);
     if ((threadIdx.x & 63) == 0) red_q[threadIdx.x >> 6] = out_q;
-    double rs = 0.0;
-    const bool books_rain = b.new_window && !num.excess;  // the excess form's rain is booked by the pre kernel
-    if (books_rain) {
-        for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += blockDim.x)
-            if (is_active[t]) rs += src_partial[t];
-        for (int off = 32; off > 0; off >>= 1) rs += __shfl_xor(rs, off);
-        if ((threadIdx.x & 63) == 0) red_s[threadIdx.x >> 6] = rs;
-        __syncthreads();
-    }
+    // the excess form's rain is booked by the pre kernel; a waiting update's by the next post kernel
+    const bool books_rain = b.new_window && !num.excess && !fused;
+    if (books_rain) rain_sum();
     __syncthreads();  // every thread has read the parities before thread 0 moves them on
     if (threadIdx.x != 0) return;
-    if (books_rain) {
-        rs = red_s[0];
-        for (int q = 1; q < POST_THREADS / 64; ++q) rs += red_s[q];
-        b.rain_window = rs * num.area * (b.window_end - b.window_start);
-    }
+    if (books_rain) b.rain_window = reduced(red_s) * num.area * (b.window_end - b.window_start);
     const long long before = b.nsteps;
+    if (fused) {  // what the pipe kernel needs of this step when it makes its update
+        b.pend = 1;
+        b.u_dt = b.dt_step;
+        b.u_want = b.refresh;
+        b.u_new_window = b.new_window;
+        b.u_scale = b.scale;
+        b.u_par = b.par;
+        b.u_nsteps = before;
+        b.u_time = b.t + b.dt_step;
+    }
     b.t += b.dt_step;
     b.nsteps = before + 1;
-    if (b.f1 > b.f0 && !failing) {  // the peak outflow over the member's steps (the run summary's peak, monitor.py)
-        double q = red_q[0];
-        for (int w = 1; w < POST_THREADS / 64; ++w) q += red_q[w];
+    if (b.f1 > b.f0 && !failing && !fused) {  // the peak outflow over the member's steps (the run summary's, monitor.py)
+        const double q = reduced(red_q);
         if (q > b.q_peak) {
             b.q_peak = q;
             b.t_peak = b.t;
@@ -2056,7 +2565,7 @@ __global__ __launch_bounds__(NT) void maxima_run_kernel(const int* __restrict__ 
     const Member& b = mem[uni(tile_member[t])];
     // sampled after the steps whose count is a multiple of `every`, as the host loop's on_step does; the member's
     // parities have moved on, so its current buffers hold the step's results
-    if (!b.stepping || b.nsteps % every != 0) return;
+    if (!b.stepping || b.fuse || b.nsteps % every != 0) return;  // a waiting update samples in the pipe kernel
     const Cur<R> s = view(B, uni(b.par));
     const R* __restrict__ d = s.d;
     const R* __restrict__ h = s.h;
@@ -2127,7 +2636,7 @@ struct w2d_handle {
     // the face momentum: steps do not store it (the face kernel recomputes it from u and the depths the last step
     // read); m_fresh says whether mx, my hold it, as after set_field or materialize; it goes to the kernels by value
     int m_fresh = 1;
-    void *qx1 = nullptr, *qy1 = nullptr, *res = nullptr, *src = nullptr;
+    void *qx1b[2] = {}, *qy1b[2] = {}, *resb[2] = {}, *src = nullptr;
     void* wave = nullptr;
     int* wet = nullptr;
     // statics
@@ -2148,6 +2657,14 @@ struct w2d_handle {
     int* tiles_scratch = nullptr;
     int nb_cells = 0;
     int *b_cell = nullptr, *b_ref = nullptr, *b_kind = nullptr;
+    std::vector<int32_t> b_cell_host, b_ref_host, b_kind_host, t_face_host, t_axis_host;
+    // the pipelined step of device-resident runs (w2d_run_pipeline): asked for, taken by the run set up last, and its
+    // per-tile flags and per-cell boundary and tally indices
+    int pipe_on = 1;
+    int run_pipe = 0;
+    int* pipe_near = nullptr;
+    int* pipe_bidx = nullptr;
+    int2* pipe_tidx = nullptr;
     float *b_d0 = nullptr, *b_h0 = nullptr;
     int n_tally = 0;
     int *t_face = nullptr, *t_axis = nullptr;
@@ -2264,14 +2781,14 @@ Bufs<R> bufs(const w2d_handle* H) {
         b.v[k] = (R*)H->vb[k];
         b.mx[k] = (R*)H->mxb[k];
         b.my[k] = (R*)H->myb[k];
+        b.res[k] = (R*)H->resb[k];
+        b.qx1[k] = (R*)H->qx1b[k];
+        b.qy1[k] = (R*)H->qy1b[k];
     }
     b.d = (R*)H->d;
     b.qx = (R*)H->qx;
     b.qy = (R*)H->qy;
     b.om = (R*)H->om;
-    b.qx1 = (R*)H->qx1;
-    b.qy1 = (R*)H->qy1;
-    b.res = (R*)H->res;
     b.src = (R*)H->src;
     return b;
 }
@@ -2295,10 +2812,10 @@ void* field_ptr(w2d_handle* H, int f) {
         case W2D_V1: return c.v1;
         case W2D_MX1: return H->mxb[1 - (H->par & 1)];
         case W2D_MY1: return H->myb[1 - (H->par & 1)];
-        case W2D_QX1: return H->qx1;
-        case W2D_QY1: return H->qy1;
+        case W2D_QX1: return c.qx1;
+        case W2D_QY1: return c.qy1;
         case W2D_SRC: return H->src;
-        case W2D_RES: return H->res;
+        case W2D_RES: return c.res;
         case W2D_OM: return c.om;
         default: return nullptr;
     }
@@ -2309,6 +2826,9 @@ void* field_ptr(w2d_handle* H, int f) {
 std::vector<void*> field_all(w2d_handle* H, int f) {
     switch (f) {
         case W2D_H: return {H->hb[0], H->hb[1]};
+        case W2D_RES: return {H->resb[0], H->resb[1]};
+        case W2D_QX1: return {H->qx1b[0], H->qx1b[1]};
+        case W2D_QY1: return {H->qy1b[0], H->qy1b[1]};
         case W2D_MX:
         case W2D_MX1: return {H->mxb[0], H->mxb[1]};
         case W2D_MY:
@@ -2561,10 +3081,10 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     if (hipGetDevice(&H->device) != hipSuccess) return bail(fail(3, "hipGetDevice failed"));
     if ((s = scratch_address(H->device, &H->scratch_dev))) return bail(s);
     const size_t n = H->ncell, es = esize(H);
-    for (void** pair : {H->hb, H->ub, H->vb, H->mxb, H->myb})
+    for (void** pair : {H->hb, H->ub, H->vb, H->mxb, H->myb, H->resb, H->qx1b, H->qy1b})
         for (int b = 0; b < 2; ++b)
             if ((s = dalloc(H, &pair[b], n * es))) return bail(s);
-    for (void** a : {&H->d, &H->qx, &H->qy, &H->om, &H->qx1, &H->qy1, &H->res, &H->src})
+    for (void** a : {&H->d, &H->qx, &H->qy, &H->om, &H->src})
         if ((s = dalloc(H, a, n * es))) return bail(s);
     if ((s = dalloc(H, &H->wave, (size_t)H->ntiles * es))) return bail(s);
     if ((s = dalloc(H, (void**)&H->wet, (size_t)H->ntiles * sizeof(int)))) return bail(s);
@@ -2871,6 +3391,9 @@ int w2d_set_boundary(w2d_handle* H, const int32_t* cell, const int32_t* ref, con
         if ((s = upload(H, &H->b_h0, h0, n))) return s;
     }
     H->nb_cells = n;
+    H->b_cell_host = c;
+    H->b_ref_host = r;
+    H->b_kind_host.assign(kind, kind + n);
     return 0;
 }
 
@@ -2983,6 +3506,8 @@ int w2d_tally_set(w2d_handle* H, const int32_t* face, const int32_t* axis, const
         HIP_TRY(hipStreamSynchronize(H->stream));
     }
     H->n_tally = n;
+    H->t_face_host = f;
+    H->t_axis_host.assign(axis, axis + n);
     return 0;
 }
 
@@ -3061,8 +3586,8 @@ int w2d_storage(w2d_handle* H, double* out) {
     int s;
     if ((s = run_align(H))) return s;
     const Cur<char> c = cur(H);
-    if (H->prec == 8) return big_sum<double>(H, H->n_store, H->store_index, (double*)c.h, (double*)H->res, out);
-    return big_sum<float>(H, H->n_store, H->store_index, (float*)c.h, (float*)H->res, out);
+    if (H->prec == 8) return big_sum<double>(H, H->n_store, H->store_index, (double*)c.h, (double*)c.res, out);
+    return big_sum<float>(H, H->n_store, H->store_index, (float*)c.h, (float*)c.res, out);
 }
 
 int w2d_maxima_set(w2d_handle* H, double arrive, double deep) {
@@ -3455,8 +3980,13 @@ int enqueue_pre(w2d_handle* H, int flush_only) {
     return 0;
 }
 
-template <class R, bool FAST>
-int enqueue_run_step(w2d_handle* H) {
+PipeArgs pipe_args(w2d_handle* H, int drain) {
+    return PipeArgs{H->run_is_active, H->pipe_near, H->pipe_bidx, H->pipe_tidx, H->b_kind, H->b_ref, H->b_d0,
+                    H->b_h0, H->t_sign, H->t_total, H->t_flow, drain};
+}
+
+template <class R>
+StepArgs<R> run_step_args(w2d_handle* H) {
     StepArgs<R> A = step_args<R>(H, 0.0, 0);
     A.members = H->run_mem;
     A.tile_member = H->run_tile_member;
@@ -3476,6 +4006,44 @@ int enqueue_run_step(w2d_handle* H) {
         A.mx_t = (R*)H->maxima[4];
         A.bnd_tile = H->run_bnd_tile;
     }
+    return A;
+}
+
+// The pipe kernel in place of the face kernel (a run whose members pipeline), then the cell kernel for the members
+// whose update this step is not left to the next step.
+template <class R, bool FAST>
+int launch_pipe_kernels(w2d_handle* H, const StepArgs<R>& A, int blocks) {
+    int s;
+    if (H->timing && (s = mark(H))) return s;
+    pipe_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A, pipe_args(H, 0));
+    HIP_TRY(hipGetLastError());
+    if (H->timing && (s = mark(H))) return s;
+    if (H->cells_threads == 512)
+        cells_kernel<R, FAST, 512><<<blocks, 512, 0, H->stream>>>(A);
+    else
+        cells_kernel<R, FAST, 256><<<blocks, 256, 0, H->stream>>>(A);
+    HIP_TRY(hipGetLastError());
+    if (H->timing && (s = mark(H))) return s;
+    return 0;
+}
+
+// The updates still waiting after the last step (pipe kernel with no face step) and their books (post kernel).
+template <class R, bool FAST>
+int enqueue_drain(w2d_handle* H) {
+    const StepArgs<R> A = run_step_args<R>(H);
+    pipe_kernel<R, FAST, 512><<<H->ntiles, 512, 0, H->stream>>>(A, pipe_args(H, 1));
+    HIP_TRY(hipGetLastError());
+    post_kernel<R><<<H->run_m, POST_THREADS, 0, H->stream>>>(H->run_mem, H->run_wave, H->run_is_active,
+                                                     H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
+                                                     H->b_d0, H->b_h0, A.b, H->t_face, H->t_axis, H->t_sign,
+                                                     H->t_total, H->t_flow, 1, H->run_glob);
+    HIP_TRY(hipGetLastError());
+    return 0;
+}
+
+template <class R, bool FAST>
+int enqueue_run_step(w2d_handle* H) {
+    const StepArgs<R> A = run_step_args<R>(H);
     const int nt = H->ntiles;
     int s;
     if (H->excess) {
@@ -3485,11 +4053,11 @@ int enqueue_run_step(w2d_handle* H) {
         HIP_TRY(hipGetLastError());
     }
     if ((s = enqueue_pre<R, FAST>(H, 0))) return s;
-    if ((s = launch_step_kernels<R, FAST>(H, A, nt))) return s;
+    if ((s = H->run_pipe ? launch_pipe_kernels<R, FAST>(H, A, nt) : launch_step_kernels<R, FAST>(H, A, nt))) return s;
     post_kernel<R><<<H->run_m, POST_THREADS, 0, H->stream>>>(H->run_mem, H->run_wave, H->run_is_active,
                                                      H->run_src_partial, H->run_num, H->b_cell, H->b_ref, H->b_kind,
                                                      H->b_d0, H->b_h0, A.b, H->t_face, H->t_axis, H->t_sign,
-                                                     H->t_total, H->t_flow);
+                                                     H->t_total, H->t_flow, 0, H->run_glob);
     if (H->run_maxima && H->run_nbnd)
         maxima_run_kernel<R><<<H->run_nbnd, NT, 0, H->stream>>>(
             H->run_bnd_list, H->run_is_active, H->run_mem, H->run_tile_member, H->nbr, (R)H->m_arrive,
@@ -3511,6 +4079,13 @@ int run_step_any(w2d_handle* H) {
     if (H->prec == 8) return enqueue_run_step<double, false>(H);
     if (H->fast) return enqueue_run_step<float, true>(H);
     return enqueue_run_step<float, false>(H);
+}
+
+int run_drain(w2d_handle* H) {
+    if (!H->run_pipe) return 0;
+    if (H->prec == 8) return enqueue_drain<double, false>(H);
+    if (H->fast) return enqueue_drain<float, true>(H);
+    return enqueue_drain<float, false>(H);
 }
 
 // The members as the device holds them, brought to the handle's parities where asked (after the handle stepped on
@@ -3688,6 +4263,56 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     H->run_nbnd = (int)bnd_list.size();
     if (bnd_list.empty()) bnd_list.push_back(0);
     if ((s = upload(H, &H->run_bnd_list, bnd_list.data(), bnd_list.size()))) return s;
+    // The pipelined step for every member whose boundary cells allow it: each tile recomputes the update of its west
+    // column and north row, so a following cell (kind 2) there must find its reference cell among the cells the tile
+    // updates (its own, its west neighbour's east column, its north neighbour's south row), and a reference is never a
+    // boundary cell; the learned closure reads the levels before the face step, so it runs the two-kernel step.
+    H->run_pipe = 0;
+    if (H->pipe_on && H->closure_n == 0) {
+        const size_t n = H->ncell;
+        std::vector<int32_t> bidx(n, -1), near(nt, 0);
+        std::vector<int2> tidx(n, make_int2(-1, -1));
+        for (int p = 0; p < H->nb_cells; ++p) bidx[H->b_cell_host[p]] = p;
+        auto nbr_at = [&](int t, int q) { return H->nbr_host[(size_t)t * 8 + q]; };  // NEIGHBOURS order
+        auto updated_by = [&](int t, int c) {  // whether tile t updates cell c
+            const int tc = c / TILE2, l = c - tc * TILE2;
+            return tc == t || (tc != 0 && tc == nbr_at(t, 3) && (l & (TILE - 1)) == TILE - 1) ||
+                   (tc != 0 && tc == nbr_at(t, 1) && l / TILE == TILE - 1);
+        };
+        std::vector<int32_t> ok(M, 1);
+        for (int p = 0; p < H->nb_cells; ++p) {
+            const int c = H->b_cell_host[p], tc = c / TILE2, l = c - tc * TILE2;
+            near[tc] |= 1;
+            if (nbr_at(tc, 4)) near[nbr_at(tc, 4)] |= 1;  // east: the cell is in its west column when l % 32 == 31
+            if (nbr_at(tc, 6)) near[nbr_at(tc, 6)] |= 1;  // south
+            if (H->b_kind_host[p] != 2) continue;
+            const int rf = H->b_ref_host[p];
+            bool fine = bidx[rf] < 0 && updated_by(tc, rf);
+            if ((l & (TILE - 1)) == TILE - 1 && nbr_at(tc, 4)) fine = fine && updated_by(nbr_at(tc, 4), rf);
+            if (l / TILE == TILE - 1 && nbr_at(tc, 6)) fine = fine && updated_by(nbr_at(tc, 6), rf);
+            if (!fine) ok[tm[tc]] = 0;
+        }
+        for (int q = 0; q < H->n_tally; ++q) {
+            const int c = H->t_face_host[q];
+            near[c / TILE2] |= 2;
+            if (H->t_axis_host[q])
+                tidx[c].y = q;
+            else
+                tidx[c].x = q;
+        }
+        for (int m = 0; m < M; ++m) {
+            mem[m].pipe = ok[m];
+            H->run_pipe |= ok[m];
+        }
+        if (H->run_pipe) {
+            if (!H->pipe_near && (s = dalloc(H, (void**)&H->pipe_near, (size_t)nt * sizeof(int)))) return s;
+            if (!H->pipe_bidx && (s = dalloc(H, (void**)&H->pipe_bidx, n * sizeof(int)))) return s;
+            if (!H->pipe_tidx && (s = dalloc(H, (void**)&H->pipe_tidx, n * sizeof(int2)))) return s;
+            HIP_TRY(hipMemcpy(H->pipe_near, near.data(), (size_t)nt * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(H->pipe_bidx, bidx.data(), n * sizeof(int), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(H->pipe_tidx, tidx.data(), n * sizeof(int2), hipMemcpyHostToDevice));
+        }
+    }
     if (H->run_mem) {
         H->owned.erase(std::remove(H->owned.begin(), H->owned.end(), (void*)H->run_mem), H->owned.end());
         (void)hipFree(H->run_mem);
@@ -3792,6 +4417,13 @@ int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
     }
     for (; done < nsteps; ++done)
         if ((s = run_step_any(H))) return s;
+    // no update waits past the call: whatever reads the state next reads it whole
+    return nsteps > 0 ? run_drain(H) : 0;
+}
+
+int w2d_run_pipeline(w2d_handle* H, int32_t on) {
+    if (!H || (on != 0 && on != 1)) return fail(1, "run_pipeline: bad arguments");
+    H->pipe_on = on;
     return 0;
 }
 
@@ -3826,6 +4458,7 @@ int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
         o.dt_min = b.dt_min;
         o.q_peak = b.q_peak;
         o.t_peak = b.t_peak;
+        o.pipelined = b.pipe;
         // rain credited pro rata in the current window (monitor.py:197-212)
         const double len = b.window_end - b.window_start;
         const double share = len > 0.0 ? (std::min(b.t, b.window_end) - b.window_start) / len : 0.0;

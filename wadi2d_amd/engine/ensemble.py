@@ -228,9 +228,10 @@ class Batch:
     member_of: np.ndarray
 
 
-def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, prepare=None, **options) -> Batch:
+def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, prepare=None, pipeline: bool = True,
+                  **options) -> Batch:  # fmt: skip
     """The mosaic model of every member, its boundary cells, source, tally and maxima, and one device run over all
-    members from t = 0 (``options`` go to the backend)."""
+    members from t = 0 (``pipeline``: the device run's pipelined step; ``options`` go to the backend)."""
     num = ens.scenarios[0].numerics
     model = Model(ens.tables, num, backend, **options)
     model.set_depth(np.zeros(ens.tables.grid.shape))
@@ -259,18 +260,18 @@ def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, pr
                                 ens.interior[m]))  # fmt: skip
     if prepare is not None:
         prepare(model)
-    run = DeviceRun(b, plans, num, maxima=maxima_every, idle=st.idle)
+    run = DeviceRun(b, plans, num, maxima=maxima_every, idle=st.idle, pipeline=pipeline)
     return Batch(model, run, faces, member_of)
 
 
 def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chunk: int = 512, tune: bool = True,
                 launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None, on_chunk=None,
-                prepare=None) -> BatchedResult:  # fmt: skip
+                prepare=None, pipeline: bool = True) -> BatchedResult:  # fmt: skip
     """Every member in one device-resident run to its end; per-member max depth (sampled every ``maxima_every``
     steps), mass error."""
     from wadi2d_amd.engine.autotune import tune_launch
 
-    bt = setup_batched(ens, backend, maxima_every, prepare)
+    bt = setup_batched(ens, backend, maxima_every, prepare, pipeline)
     model, run, faces, member_of = bt.model, bt.run, bt.faces, bt.member_of
     b = model.backend
     plans = run.plans

@@ -49,7 +49,8 @@ class RunNumerics(ctypes.Structure):
 
 class MemberState(ctypes.Structure):
     _fields_ = [("t", D), ("dt", D), ("rain", D), ("nsteps", I64), ("done", I32), ("failed", I32),
-                ("refresh_due", I32), ("dry", I32), ("dt_min", D), ("q_peak", D), ("t_peak", D)]  # fmt: skip
+                ("refresh_due", I32), ("dry", I32), ("dt_min", D), ("q_peak", D), ("t_peak", D),
+                ("pipelined", I32)]  # fmt: skip
 
 
 _SIGS = {
@@ -112,6 +113,7 @@ _SIGS = {
     "w2d_run_read": [VP, ctypes.POINTER(MemberState), I32],
     "w2d_run_idle": [VP, VP],
     "w2d_run_pause": [VP, VP, I32],
+    "w2d_run_pipeline": [VP, I32],
     "w2d_active_tiles": [VP, VP, ctypes.POINTER(I32)],
     "w2d_tally_totals": [VP, VP],
     "w2d_copy_probe": [I64, I32, ctypes.POINTER(D)],
@@ -451,6 +453,10 @@ class HipBackend:
         """Hold each member of the run once its clock reaches t_stop[m] (inf: run on)."""
         ts = _c(np.asarray(t_stop, np.float64).ravel(), np.float64)
         check(api().w2d_run_pause(self.h, ts.ctypes.data, int(ts.size)))
+
+    def run_pipeline(self, on: bool) -> None:
+        """Whether the next run set up takes the pipelined step (the default) or the two-kernel one."""
+        check(api().w2d_run_pipeline(self.h, int(bool(on))))
 
     def run_idle(self) -> np.ndarray:
         out = np.zeros(self.grid.ntiles, np.int32)
