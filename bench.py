@@ -45,6 +45,9 @@ HBM_PEAK = 8.0e12  # MI355X HBM3E, B/s
 FACE_BYTES = 144
 CELL_BYTES = 52   # h,qx1,qy1,res,src 20 + Hc 4 + level pair 8 + writes qx,qy,h,d,res 20
 STEP_BYTES = 176  # the whole step as one pass: 140 read + 36 written
+# The pipelined device loop's pipe kernel makes, per launch, the last step's cell update and this step's faces: one
+# step's work in one pass (STEP_BYTES per cell); after a step whose update the cell kernel made it has no update to
+# make and does the face step alone (FACE_BYTES).
 
 
 def domain_cells_of(tables) -> int:
@@ -145,6 +148,7 @@ def main() -> int:
     st = model.stepper
     t0 = time.time()
     hot = args.hot
+    made = 0
     if args.loop == "host":
         while st.t < hot - 1e-9:
             drv.run(min(st.t + 600.0, hot))
@@ -186,12 +190,15 @@ def main() -> int:
         bal = run.balance(0, b.tally_volume())
         # per-kernel times in a pass of their own: HIP events around each kernel (the loop then runs uncaptured)
         faces_ms, cells_ms, launches = 0.0, 0.0, 0
+        made = 0  # of those launches, the pipe kernel's that made a cell update
         if args.events == "on" and args.event_steps > 0:
             b.timing_read()
             b.timing(True)
+            made0 = state.pipelined_updates
             state = run.steps(args.event_steps, args.graph)[0]
             b.timing(False)
             faces_ms, cells_ms, launches = b.timing_read()
+            made = state.pipelined_updates - made0
             assert launches == args.event_steps and not state.done, (launches, state.done)
         # the run loop's own sampling: the running maxima every 4 steps, a run of its own from this state
         maxima4 = None
@@ -218,8 +225,11 @@ def main() -> int:
     value = domain_cells * steps / wall
     face_s = faces_ms / 1e3 / launches
     cell_s = cells_ms / 1e3 / launches
-    kernel = "faces_kernel"  # the dominant kernel
-    face_achieved = FACE_BYTES * tiles * 1024 / face_s if face_s > 0 else 0.0
+    # the dominant kernel: the pipe kernel in the pipelined device loop, else the face kernel
+    pipelined = args.loop == "device" and args.pipeline == "on"
+    kernel = "pipe_kernel" if pipelined else "faces_kernel"
+    alg_bytes = (made * STEP_BYTES + (launches - made) * FACE_BYTES) / launches if pipelined else FACE_BYTES
+    face_achieved = alg_bytes * tiles * 1024 / face_s if face_s > 0 else 0.0
     log(f"[bench] timed {steps} steps in {wall:.3f} s ({wall / steps * 1e3:.3f} ms/step), simulated "
         f"{tsim:.0f} s; faces {face_s * 1e3:.3f} ms, cells {cell_s * 1e3:.3f} ms per launch; "
         f"active tiles {tiles}; mass error {bal.relative:.2e}")
@@ -253,8 +263,11 @@ def main() -> int:
             "variant": args.variant,
             "loop": args.loop,
             "hbm_frac_step": STEP_BYTES * value / HBM_PEAK,
-            "faces_ms": face_s * 1e3,
+            "pipeline": pipelined,
+            "faces_ms": face_s * 1e3,  # the pipe kernel's when pipelined
             "cells_ms": cell_s * 1e3,
+            "pipe_updates_share": made / launches if pipelined else None,
+            "pipe_bytes_per_cell": alg_bytes,
             "kernel_times": (f"HIP events, a pass of {launches} steps after the timed ones" if args.loop == "device"
                              else "HIP events inside the timed steps"),
             "timed_path": ("captured graphs of %d steps, wall time" % args.graph if args.loop == "device"

@@ -180,6 +180,7 @@ typedef struct w2d_member_state {
     double q_peak, t_peak; /* the largest discharge out through the member's tally faces after a step of this run
                             * (the FP64 sum of sign * q), and the time it was reached */
     int32_t pipelined;     /* the member takes the pipelined step (w2d_run_pipeline) */
+    int64_t pipelined_updates; /* cell updates of this run made by the pipe kernel (the rest by the cell kernel) */
 } w2d_member_state;
 /* With the excess source (w2d_source_losses), a device-resident run takes (d0, d1, denom) per window, in the order of
  * the scales (3 nwindows doubles, nwindows = nscales), before w2d_run_setup: its windows' rates are formed on the

@@ -697,6 +697,7 @@ struct Member {
     int pipe, fuse, pend;
     int u_want, u_new_window, u_par;
     long long u_nsteps;
+    long long u_made;  // updates the pipe kernel has made for this member (a count for the host)
     double u_dt, u_scale, u_time;
 };
 
@@ -1275,6 +1276,7 @@ __global__ __launch_bounds__(NTH, 6) void pipe_kernel(StepArgs<R> A, PipeArgs G)
     __shared__ __align__(16) R s_qx[LSIZE];  // the discharges before the limit, then after it
     __shared__ __align__(16) R s_qy[LSIZE];
     __shared__ __align__(16) R s_r[LSIZE];  // the positivity ratio, then the wet share
+    __shared__ R s_mx[TILE2], s_my[TILE2];   // the own faces' momenta (x, y), from the staged depths
     __shared__ int nb[9], live[9];
     __shared__ R red[NTH / 64];
     __shared__ int any_wet;
@@ -1366,14 +1368,13 @@ __global__ __launch_bounds__(NTH, 6) void pipe_kernel(StepArgs<R> A, PipeArgs G)
         const R have = pmax(s_h[L], R(0)) * area;
         s_r[L] = out * dt_u > have ? dv<FAST>(have, out * dt_u) : R(1);
     }
-    // 2. the own faces' momenta from the depths the last face step read
-    R mxo[RO], myo[RO];
+    // 2. the own faces' momenta from the depths the last face step read, into LDS (the face loop reads them)
 #pragma unroll
     for (int r = 0; r < RO; ++r) {
         const int j = (threadIdx.x >> 5) + r * (NTH / TILE);
         const int L = lds_of(j, i);
-        mxo[r] = R(0.5) * (s_h[L - 1] + s_h[L]) * dx * s_u[L];
-        myo[r] = R(0.5) * (s_h[L - LS] + s_h[L]) * dx * s_v[L];
+        s_mx[j * TILE + i] = R(0.5) * (s_h[L - 1] + s_h[L]) * dx * s_u[L];
+        s_my[j * TILE + i] = R(0.5) * (s_h[L - LS] + s_h[L]) * dx * s_v[L];
     }
     __syncthreads();
     // 3. limited discharges: x faces of cells [-1, 31] x [-1, 32], y faces of cells [-1, 32] x [-1, 31]
@@ -1612,7 +1613,8 @@ __global__ __launch_bounds__(NTH, 6) void pipe_kernel(StepArgs<R> A, PipeArgs G)
             omn = s_r[L - LS];
             omo = s_r[L];
         }
-        const R m = face_cell<R, FAST, false>(A, sf, fx, fy, L, c, mxo[r], myo[r], zxw, zxo, zyn, zyo, omw, omn, omo,
+        const R m = face_cell<R, FAST, false>(A, sf, fx, fy, L, c, s_mx[j * TILE + i], s_my[j * TILE + i], zxw, zxo,
+                                              zyn, zyo, omw, omn, omo,
                                               dt_f, want_wave);
         w = m > w ? m : w;
     }
@@ -2478,6 +2480,7 @@ __global__ __launch_bounds__(POST_THREADS) void post_kernel(Member* __restrict__
                 }
             }
             b.pend = 0;
+            if (!drain) b.u_made += 1;  // the drain's update follows the last step
         }
         __syncthreads();
         if (!step) return;
@@ -4459,6 +4462,7 @@ int w2d_run_read(w2d_handle* H, w2d_member_state* out, int32_t M) {
         o.q_peak = b.q_peak;
         o.t_peak = b.t_peak;
         o.pipelined = b.pipe;
+        o.pipelined_updates = b.u_made;
         // rain credited pro rata in the current window (monitor.py:197-212)
         const double len = b.window_end - b.window_start;
         const double share = len > 0.0 ? (std::min(b.t, b.window_end) - b.window_start) / len : 0.0;

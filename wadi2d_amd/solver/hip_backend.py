@@ -50,7 +50,7 @@ class RunNumerics(ctypes.Structure):
 class MemberState(ctypes.Structure):
     _fields_ = [("t", D), ("dt", D), ("rain", D), ("nsteps", I64), ("done", I32), ("failed", I32),
                 ("refresh_due", I32), ("dry", I32), ("dt_min", D), ("q_peak", D), ("t_peak", D),
-                ("pipelined", I32)]  # fmt: skip
+                ("pipelined", I32), ("pipelined_updates", I64)]  # fmt: skip
 
 
 _SIGS = {
