@@ -506,9 +506,12 @@ def measured_traffic(args, tiles: int, kernel: str):
         return None
     with open(path) as f:
         rec = json.load(f)
-    if tiles != 16641 or rec["launch"]["faces_threads"] != args.faces_threads or kernel not in rec["per_launch"]:
+    if tiles != 16641 or rec["launch"]["faces_threads"] != args.faces_threads:
         return None
-    return rec["per_launch"][kernel]["bytes"]
+    for per in (rec["per_launch"], rec.get("two_kernel_step", {}).get("per_launch", {})):
+        if kernel in per:
+            return per[kernel]["bytes"]
+    return None
 
 
 def cpu_model() -> str:
