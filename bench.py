@@ -4,15 +4,16 @@ Workload: the full-extent plain FP32 step, 4096 x 4096 cells at 30 m (16,777,216
 12-row sub-grid tables, a Gaussian storm cell on a triangular 3 h hyetograph, four free outflow segments
 (``wadi2d_amd.synth.full_extent_case``). The run is taken hot to t = 1.5 h (the storm's peak) through the same
 window loop a run uses, then W steps of warm-up and exactly K timed steps. A step is what the stepper does per
-step: both kernels, the boundary cells, the FP64 boundary tally, a wave-speed reduction every 8 steps, the
-active-tile check every 16 steps and a source refresh every 30 s of simulated time. By default the loop runs on
-the device (engine/device_run.py: captured graphs of 64 steps, no host in the loop); ``--loop host`` drives the same
-steps from Python through the backend protocol.
+step: the face step and the cell update, the boundary cells, the FP64 boundary tally, a wave-speed reduction every
+8 steps, the active-tile check every 16 steps and a source refresh every 30 s of simulated time. By default the loop
+runs on the device (engine/device_run.py: captured graphs of 64 steps, no host in the loop) with the pipelined step
+(a step's cell update made by the next step's kernel with its faces; ``--pipeline off`` for the two-kernel step);
+``--loop host`` drives the same steps from Python through the backend protocol.
 
 Printed: one JSON line. ``value`` counts domain cells (padding excluded) times steps over the wall time of the
-timed region (graph-replayed steps, no events), with the inputs resident in HBM. ``roofline`` is the face kernel's
-(the dominant one): its algorithmic bytes per launch over its mean launch time, from HIP events on the library's
-stream in a pass of its own after the timed steps. ``config.maxima_every_4`` is a further pass with the running
+timed region (graph-replayed steps, no events), with the inputs resident in HBM. ``roofline`` is the dominant
+kernel's (the pipe kernel, or the face kernel with the pipeline off): its algorithmic bytes per launch over its mean
+launch time, from HIP events on the library's stream in a pass of its own after the timed steps. ``config.maxima_every_4`` is a further pass with the running
 maxima sampled every 4 steps, as a run samples them by default. ``cpu_baseline`` is the oracle's float
 instantiation (the reference's float32 statement order, OpenMP over tiles) on the host's cores, at least 100 steps
 from the same hot state, with the CPU's model and its nanoseconds per cell-update.
