@@ -2665,6 +2665,7 @@ struct w2d_handle {
     // per-tile flags and per-cell boundary and tally indices
     int pipe_on = 1;
     int run_pipe = 0;
+    std::vector<int32_t> pipe_ok;  // per member of the run set up last: its boundary cells allow the pipelined step
     int* pipe_near = nullptr;
     int* pipe_bidx = nullptr;
     int2* pipe_tidx = nullptr;
@@ -3064,6 +3065,9 @@ int w2d_create(const w2d_tables* T, const w2d_params* p, int32_t precision, int3
     *out = nullptr;
     H->prec = precision;
     H->fast = precision == W2D_F32 && (variant & 1) == W2D_FAST;
+    // the pipelined step by default in FP32; in FP64 its LDS (80 KB a workgroup) leaves one workgroup per CU and the
+    // two-kernel step is faster (it can still be asked for)
+    H->pipe_on = precision == W2D_F32;
     H->face_cache = (variant & W2D_NO_FACE_CACHE) == 0;
     H->ntiles = T->ntiles;
     H->rows = T->rows;
@@ -4271,7 +4275,8 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     // updates (its own, its west neighbour's east column, its north neighbour's south row), and a reference is never a
     // boundary cell; the learned closure reads the levels before the face step, so it runs the two-kernel step.
     H->run_pipe = 0;
-    if (H->pipe_on && H->closure_n == 0) {
+    H->pipe_ok.assign(M, 0);
+    if (H->closure_n == 0) {  // the indices are set up whether asked or not: w2d_run_pipeline may switch mid-run
         const size_t n = H->ncell;
         std::vector<int32_t> bidx(n, -1), near(nt, 0);
         std::vector<int2> tidx(n, make_int2(-1, -1));
@@ -4303,11 +4308,12 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
             else
                 tidx[c].x = q;
         }
+        H->pipe_ok = ok;
         for (int m = 0; m < M; ++m) {
-            mem[m].pipe = ok[m];
-            H->run_pipe |= ok[m];
+            mem[m].pipe = H->pipe_on && ok[m];
+            H->run_pipe |= mem[m].pipe;
         }
-        if (H->run_pipe) {
+        {
             if (!H->pipe_near && (s = dalloc(H, (void**)&H->pipe_near, (size_t)nt * sizeof(int)))) return s;
             if (!H->pipe_bidx && (s = dalloc(H, (void**)&H->pipe_bidx, n * sizeof(int)))) return s;
             if (!H->pipe_tidx && (s = dalloc(H, (void**)&H->pipe_tidx, n * sizeof(int2)))) return s;
@@ -4427,6 +4433,23 @@ int w2d_run_steps(w2d_handle* H, int32_t nsteps, int32_t graph_steps) {
 int w2d_run_pipeline(w2d_handle* H, int32_t on) {
     if (!H || (on != 0 && on != 1)) return fail(1, "run_pipeline: bad arguments");
     H->pipe_on = on;
+    if (!H->run_m || (int)H->pipe_ok.size() != H->run_m) return 0;
+    // a run set up already takes it from its next step (run_steps returned with no update waiting)
+    std::vector<Member> mem;
+    int s;
+    if ((s = members_read(H, mem))) return s;
+    int any = 0;
+    for (int m = 0; m < H->run_m; ++m) {
+        mem[m].pipe = on && H->pipe_ok[m];
+        any |= mem[m].pipe;
+    }
+    if ((s = members_write(H, mem))) return s;
+    if (any != H->run_pipe && H->graph) {  // a captured graph holds the other step's launches
+        (void)hipGraphExecDestroy(H->graph);
+        H->graph = nullptr;
+        H->graph_steps = 0;
+    }
+    H->run_pipe = any;
     return 0;
 }
 

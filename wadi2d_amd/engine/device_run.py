@@ -63,10 +63,11 @@ def window_terms(rain: Rain, t0: float, t_end: float, every: float) -> np.ndarra
 
 class DeviceRun:
     def __init__(self, backend, plans: list[MemberPlan], num, source_every_s: float = 30.0, maxima: int = 0,
-                 idle: np.ndarray | None = None, pipeline: bool = True) -> None:  # fmt: skip
+                 idle: np.ndarray | None = None, pipeline: bool | None = None) -> None:  # fmt: skip
         """``maxima``: sample the running maxima after every step whose count is a multiple of it (0: never).
         ``pipeline``: take the pipelined step (a step's cell update made by the next step's kernel, with its faces)
-        where the members allow it; the fields are the same either way."""
+        where the members allow it, or not; None keeps the library's default (on in FP32, off in FP64). The fields
+        are the same either way."""
         self.b = backend
         self.plans = plans
         self.area = backend.grid.dx ** 2
@@ -91,7 +92,8 @@ class DeviceRun:
                 rows = [p.rain.pixel_terms(t0, t1) for t0, t1 in window_bounds(p.t0, p.t_end, source_every_s)]
                 backend.run_pixel_terms(np.asarray(rows).reshape(len(rows), 2, -1))
         rn = RunNumerics(num.cfl, num.dt_max, num.dt_min, num.wave_floor, num.dt_every, num.active_every, int(maxima))
-        backend.run_pipeline(pipeline)
+        if pipeline is not None:
+            backend.run_pipeline(pipeline)
         backend.run_setup(members, self.scales, rn, idle)
         self.state = backend.run_read()
         self.storage0 = [self._storage(p) for p in plans]

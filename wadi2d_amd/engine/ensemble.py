@@ -228,8 +228,8 @@ class Batch:
     member_of: np.ndarray
 
 
-def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, prepare=None, pipeline: bool = True,
-                  **options) -> Batch:  # fmt: skip
+def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, prepare=None,
+                  pipeline: bool | None = None, **options) -> Batch:  # fmt: skip
     """The mosaic model of every member, its boundary cells, source, tally and maxima, and one device run over all
     members from t = 0 (``pipeline``: the device run's pipelined step; ``options`` go to the backend)."""
     num = ens.scenarios[0].numerics
@@ -266,10 +266,10 @@ def setup_batched(ens: Ensemble, backend: str = "hip", maxima_every: int = 1, pr
 
 def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chunk: int = 512, tune: bool = True,
                 launch: tuple[int, int] | None = None, maxima_every: int = 1, log=None, on_chunk=None,
-                prepare=None, pipeline: bool = True) -> BatchedResult:  # fmt: skip
+                prepare=None, pipeline: bool | None = None) -> BatchedResult:  # fmt: skip
     """Every member in one device-resident run to its end; per-member max depth (sampled every ``maxima_every``
     steps), mass error."""
-    from wadi2d_amd.engine.autotune import tune_launch
+    from wadi2d_amd.engine.autotune import apply_launch, tune_launch
 
     bt = setup_batched(ens, backend, maxima_every, prepare, pipeline)
     model, run, faces, member_of = bt.model, bt.run, bt.faces, bt.member_of
@@ -279,7 +279,7 @@ def run_batched(ens: Ensemble, backend: str = "hip", graph_steps: int = 64, chun
     t0 = time.perf_counter()
     tuning = None
     if launch is not None:
-        b.set_launch(*launch)
+        apply_launch(b, launch)
     elif tune:
         launch, tuning = tune_launch(run, graph_steps=graph_steps)
     state = run.state
