@@ -99,3 +99,55 @@ def test_pipelined_ensemble_with_a_held_member_is_the_two_kernel_one(backend):
     for m in range(2):
         _same_state(sa[m], sb[m])
     _same_model(a.model, b.model, ens.tables.grid)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("edge", [False, True])
+def test_following_cells_keep_the_pipeline_only_where_each_tile_updates_their_reference(edge):
+    """A column of following boundary cells (kind 2) inside the domain: in the middle of a tile, each following its
+    west neighbour (every tile that updates the cell updates its reference: the pipelined step applies), or at a
+    tile's east column following its east neighbour (the next tile's west column, which the cell's own tile does not
+    update: the member takes the two-kernel step). Either way the device loop is the host loop bit for bit
+    (cells.cu:95-111 after every update)."""
+    import dataclasses
+
+    from wadi2d_amd.solver.backend import BoundaryCells
+    from wadi2d_amd.solver.state import state_index
+
+    t_end = 1800.0
+    runs = []
+    steps = 0
+    for loop in ("host", "device"):
+        tables, bset, rain, m = _valley_setup("hip-exact")
+        ny, _ = tables.grid.shape
+        j = np.arange(2, ny - 2)
+        i = np.full(j.size, 63 if edge else 47)  # column 31 or 15 of the second tile column
+        cell = state_index(tables.grid, j, i).astype(np.int32)
+        ref = state_index(tables.grid, j, i + (1 if edge else -1)).astype(np.int32)
+        extra = BoundaryCells(cell, ref, np.full(j.size, 2, np.int32), np.full(j.size, -0.05, np.float32),
+                              np.zeros(j.size, np.float32))  # fmt: skip
+        c = bset.cells
+        both = BoundaryCells(*(np.concatenate((getattr(c, f), getattr(extra, f))) for f in
+                               ("cell", "ref", "kind", "d0", "h0")))  # fmt: skip
+        mask = bset.mask.copy()
+        mask[j, i] = True
+        bset2 = dataclasses.replace(bset, cells=both, mask=mask)
+        drv = Driver(m, bset2, rain, maxima_every=1)
+        if loop == "host":
+            drv.run(t_end)
+            steps = m.stepper.nsteps
+            runs.append((m, None))
+        else:  # exactly the host's steps (one more would retire the member)
+            run = DeviceRun(m.backend, [_device_plan(m, bset2, rain, t_end)], m.num, maxima=1, idle=m.stepper.idle)
+            s = run.steps(steps, 16)[0]
+            runs.append((m, s))
+    (host, _), (dev, s) = runs
+    print(f"edge {edge}: pipelined {s.pipelined}, {s.nsteps}/{host.stepper.nsteps} steps", flush=True)
+    assert s.pipelined == (0 if edge else 1)
+    assert s.nsteps == host.stepper.nsteps and s.t == host.stepper.t
+    for f in FIELDS:
+        assert np.array_equal(dev.field(f), host.field(f)), f
+    for which in range(5):
+        x = from_state(host.grid, host.backend.maxima_get(which))
+        y = from_state(dev.grid, dev.backend.maxima_get(which))
+        assert np.array_equal(x, y, equal_nan=True), which
