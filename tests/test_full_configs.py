@@ -70,23 +70,31 @@ def test_batched_ensemble_of_64_members_against_fp64():
     assert max(abs(m) for m in ref.mass_error) <= 1e-12
 
 
-def test_learned_closure_on_32_held_out_members_in_the_batched_launch():
+def test_learned_closure_on_32_held_out_members_in_the_batched_launch(capsys):
     """The closure inside the batched launch at its stated size: a network answering zero leaves every member's
-    maxima bit for bit the plain launch's; the trained network moves them (on crest faces) and keeps the books."""
+    maxima bit for bit the plain launch's; the trained network moves them (on crest faces) and keeps the books. The
+    members' build (minutes of CPU work) reports each member as it comes, past the output capture."""
+
+    def say(line: str) -> None:
+        with capsys.disabled():
+            print(line, flush=True)
+
     from wadi2d_amd.closure.evaluate import coarse_run
     from wadi2d_amd.closure.model import ClosureModel
 
     from wadi2d_amd.closure.data import held_out_members
 
     t0 = time.time()
-    pairs = held_out_members(32, min(16, os.cpu_count() or 1), log=lambda s: print(s, flush=True))
+    pairs = held_out_members(32, min(16, os.cpu_count() or 1), log=say)
     coarse, faces = [c for c, _ in pairs], [f for _, f in pairs]
     built = time.time() - t0
     path = os.path.join(os.path.dirname(os.path.dirname(__file__)), "wadi2d_amd", "closure", "trained.npz")
     trained = ClosureModel.load(path)
     zero = trained.with_(params=np.zeros_like(trained.params))  # every output 0: conveyance factor 1, no invert rise
     plain, ens = coarse_run(coarse, faces, None)
+    say(f"[closure] plain launch {plain.wall_s:.1f} s")
     nothing, _ = coarse_run(coarse, faces, zero)
+    say(f"[closure] zero network {nothing.wall_s:.1f} s")
     learned, _ = coarse_run(coarse, faces, trained)
     moved = max(float(np.abs(a - b)[f.tables.grid.mask].max()) for a, b, f in zip(learned.max_depth, plain.max_depth,
                                                                                   faces))  # fmt: skip

@@ -151,3 +151,39 @@ def test_following_cells_keep_the_pipeline_only_where_each_tile_updates_their_re
         x = from_state(host.grid, host.backend.maxima_get(which))
         y = from_state(dev.grid, dev.backend.maxima_get(which))
         assert np.array_equal(x, y, equal_nan=True), which
+
+
+def _wet_valley_run(backend, pipeline, t_end, chunks):
+    """The valley wet all over (0.3 m mean depth) under a storm on every cell: every tile stays listed."""
+    from wadi2d_amd.engine.forcing import Rain, uniform_storm
+
+    tables, bset, _, m = _valley_setup(backend)
+    rain = Rain(tables, bset, uniform_storm(40.0, 1800.0), np.ones(tables.grid.shape))
+    m.set_depth(np.full(tables.grid.shape, 0.3))
+    Driver(m, bset, rain, maxima_every=1)
+    run = DeviceRun(m.backend, [_device_plan(m, bset, rain, t_end)], m.num, maxima=1, idle=m.stepper.idle,
+                    pipeline=pipeline)  # fmt: skip
+    for n, graph in chunks:
+        s = run.steps(n, graph)
+    return m, s[0]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("backend", ["hip-exact", "hip"])
+def test_window_ends_keep_the_pipeline_when_every_tile_is_listed(backend):
+    """With every tile listed a window's end cannot change the list, so the step that ends it leaves its update to the
+    next step's pipe kernel like any other: only the steps before a check, the run's last step and each chunk's last
+    (whose update run_steps makes before it returns) take another path. Still the two-kernel run bit for bit."""
+    t_end = 1500.0
+    chunks = [(90, 16), (33, 0), (5000, 64)]
+    a, sa = _wet_valley_run(backend, True, t_end, chunks)
+    b, sb = _wet_valley_run(backend, False, t_end, chunks)
+    n, every = sa.nsteps, a.num.active_every
+    windows = int(t_end // 30.0)
+    print(f"{backend}: {n} steps, {sa.pipelined_updates} pipelined updates, checks every {every}, {windows} windows",
+          flush=True)  # fmt: skip
+    assert sa.done and sb.done
+    # a window's end left to the cell kernel would allow at most n - n // every - (windows - 1) pipelined updates
+    assert sa.pipelined_updates >= n - n // every - len(chunks) - 2 > n - n // every - (windows - 1)
+    _same_state(sa, sb)
+    _same_model(a, b, a.grid)

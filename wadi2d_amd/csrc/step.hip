@@ -1951,11 +1951,15 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.stepping = 1;
     // this step's update is left to the next step's pipe kernel unless the next step reads the wet flags it leaves (a
     // check follows a step whose count is a multiple of active_every) or may change the list or the source (a window
-    // end, the run's end or a hold: the clock after this step is post_kernel's t + dt_step, the same sum)
+    // end, the run's end or a hold: the clock after this step is post_kernel's t + dt_step, the same sum). A window's
+    // end keeps the pipeline when the next window cannot change anything this update leaves: the list only grows
+    // there, and a member with every tile listed cannot; the excess source rewrites the cells' rates before the next
+    // pipe kernel; and a step that opened its window books that window's rain after its update
     if (b.pipe) {
         const double t1 = b.t + b.dt_step;
-        b.fuse = (b.nsteps % num.active_every) != 0 && !(t1 >= b.window_end - 1e-9) && !(t1 >= b.t_end - 1e-9) &&
-                 !(t1 >= b.t_stop);
+        const bool window_kept = !num.excess && !b.new_window && b.nlist == b.tile1 - b.tile0;
+        b.fuse = (b.nsteps % num.active_every) != 0 && (window_kept || !(t1 >= b.window_end - 1e-9)) &&
+                 !(t1 >= b.t_end - 1e-9) && !(t1 >= b.t_stop);
     }
 }
 
