@@ -153,12 +153,12 @@ def test_following_cells_keep_the_pipeline_only_where_each_tile_updates_their_re
         assert np.array_equal(x, y, equal_nan=True), which
 
 
-def _wet_valley_run(backend, pipeline, t_end, chunks):
-    """The valley wet all over (0.3 m mean depth) under a storm on every cell: every tile stays listed."""
+def _wet_valley_run(backend, pipeline, t_end, chunks, storm_s):
+    """The valley wet all over (0.3 m mean depth) under a storm on every cell for storm_s seconds."""
     from wadi2d_amd.engine.forcing import Rain, uniform_storm
 
     tables, bset, _, m = _valley_setup(backend)
-    rain = Rain(tables, bset, uniform_storm(40.0, 1800.0), np.ones(tables.grid.shape))
+    rain = Rain(tables, bset, uniform_storm(40.0 * storm_s / 1800.0, storm_s), np.ones(tables.grid.shape))
     m.set_depth(np.full(tables.grid.shape, 0.3))
     Driver(m, bset, rain, maxima_every=1)
     run = DeviceRun(m.backend, [_device_plan(m, bset, rain, t_end)], m.num, maxima=1, idle=m.stepper.idle,
@@ -169,21 +169,25 @@ def _wet_valley_run(backend, pipeline, t_end, chunks):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("backend", ["hip-exact", "hip"])
-def test_window_ends_keep_the_pipeline_when_every_tile_is_listed(backend):
-    """With every tile listed a window's end cannot change the list, so the step that ends it leaves its update to the
-    next step's pipe kernel like any other: only the steps before a check, the run's last step and each chunk's last
-    (whose update run_steps makes before it returns) take another path. Still the two-kernel run bit for bit."""
+@pytest.mark.parametrize("backend,storm_s", [("hip-exact", 1800.0), ("hip", 1800.0), ("hip-exact", 600.0)])
+def test_window_ends_and_checks_keep_the_pipeline_when_the_list_cannot_change(backend, storm_s):
+    """With every tile listed a window's end cannot change the list, and while it rains on every tile a check cannot
+    either (each tile is kept whatever its wet flag says): the step before either leaves its update to the next step's
+    pipe kernel like any other, and only each chunk's last step (whose update run_steps makes before it returns) and
+    the run's last take another path. A storm that stops at 600 s turns the checks' rule off mid-run (and with it,
+    as tiles dry and leave, the windows'). Either way the two-kernel run bit for bit."""
     t_end = 1500.0
     chunks = [(90, 16), (33, 0), (5000, 64)]
-    a, sa = _wet_valley_run(backend, True, t_end, chunks)
-    b, sb = _wet_valley_run(backend, False, t_end, chunks)
+    a, sa = _wet_valley_run(backend, True, t_end, chunks, storm_s)
+    b, sb = _wet_valley_run(backend, False, t_end, chunks, storm_s)
     n, every = sa.nsteps, a.num.active_every
-    windows = int(t_end // 30.0)
-    print(f"{backend}: {n} steps, {sa.pipelined_updates} pipelined updates, checks every {every}, {windows} windows",
-          flush=True)  # fmt: skip
+    print(f"{backend}, storm {storm_s:.0f} s: {n} steps, {sa.pipelined_updates} pipelined updates, checks every "
+          f"{every}", flush=True)  # fmt: skip
     assert sa.done and sb.done
-    # a window's end left to the cell kernel would allow at most n - n // every - (windows - 1) pipelined updates
-    assert sa.pipelined_updates >= n - n // every - len(chunks) - 2 > n - n // every - (windows - 1)
+    if storm_s >= t_end:
+        # a check or a window's end left to the cell kernel would allow at most n - n // every pipelined updates
+        assert sa.pipelined_updates >= n - len(chunks) - 2 > n - n // every
+    else:  # the checks after the storm (some 15 of them) take the cell kernel
+        assert sa.pipelined_updates < n - 5
     _same_state(sa, sb)
     _same_model(a, b, a.grid)

@@ -695,6 +695,8 @@ struct Member {
     // (`fuse`) leaves its cell update to the next step's pipe kernel; `pend` says an update waits there, with its
     // step's parameters (`u_*`: length, wave step, new window and scale, parity, steps before it, clock after it).
     int pipe, fuse, pend;
+    int kept_rain, kept_dry;  // the member's tiles a check keeps whatever their wet flags (a source while it rains, a
+                              // boundary cell)
     int u_want, u_new_window, u_par;
     long long u_nsteps;
     long long u_made;  // updates the pipe kernel has made for this member (a count for the host)
@@ -1951,14 +1953,18 @@ __device__ void ctl_begin(Member& b, const double* __restrict__ scales, const Ru
     b.stepping = 1;
     // this step's update is left to the next step's pipe kernel unless the next step reads the wet flags it leaves (a
     // check follows a step whose count is a multiple of active_every) or may change the list or the source (a window
-    // end, the run's end or a hold: the clock after this step is post_kernel's t + dt_step, the same sum). A window's
-    // end keeps the pipeline when the next window cannot change anything this update leaves: the list only grows
-    // there, and a member with every tile listed cannot; the excess source rewrites the cells' rates before the next
-    // pipe kernel; and a step that opened its window books that window's rain after its update
+    // end, the run's end or a hold: the clock after this step is post_kernel's t + dt_step, the same sum). Unless the
+    // next step cannot change anything this update leaves. A member with every tile listed: a window's end can only
+    // add tiles, so it changes nothing (but a step that opened its window books that window's rain after its update);
+    // a check changes nothing when every tile is also kept whatever its wet flag says (a source tile while it rains,
+    // a boundary tile). The excess source rewrites the cells' rates, and the tiles it keeps, before the next step.
     if (b.pipe) {
         const double t1 = b.t + b.dt_step;
-        const bool window_kept = !num.excess && !b.new_window && b.nlist == b.tile1 - b.tile0;
-        b.fuse = (b.nsteps % num.active_every) != 0 && (window_kept || !(t1 >= b.window_end - 1e-9)) &&
+        const int own = b.tile1 - b.tile0;
+        const bool listed = !num.excess && b.nlist == own;
+        const bool window_kept = listed && !b.new_window;
+        const bool check_kept = listed && (b.scale != 0.0 ? b.kept_rain : b.kept_dry) == own;
+        b.fuse = (check_kept || (b.nsteps % num.active_every) != 0) && (window_kept || !(t1 >= b.window_end - 1e-9)) &&
                  !(t1 >= b.t_end - 1e-9) && !(t1 >= b.t_stop);
     }
 }
@@ -4231,6 +4237,11 @@ int w2d_run_setup(w2d_handle* H, const w2d_member* members, int32_t M, const dou
     };
     if ((s = ranges(H->b_cell, H->nb_cells, true, "boundary cells"))) return s;
     if ((s = ranges(H->t_face, H->n_tally, false, "tally faces"))) return s;
+    for (int m = 0; m < M; ++m)
+        for (int t = members[m].tile0; t < members[m].tile1; ++t) {
+            mem[m].kept_rain += has_source[t] || bnd_tile[t];
+            mem[m].kept_dry += bnd_tile[t];
+        }
     // each member's neighbours are its own tiles or the void tile: members never touch each other's tiles
     for (int m = 0; m < M; ++m)
         for (int t = members[m].tile0; t < members[m].tile1; ++t)
