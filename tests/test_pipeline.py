@@ -153,14 +153,19 @@ def test_following_cells_keep_the_pipeline_only_where_each_tile_updates_their_re
         assert np.array_equal(x, y, equal_nan=True), which
 
 
-def _wet_valley_run(backend, pipeline, t_end, chunks, storm_s):
+def _wet_valley(backend, storm_s):
     """The valley wet all over (0.3 m mean depth) under a storm on every cell for storm_s seconds."""
     from wadi2d_amd.engine.forcing import Rain, uniform_storm
 
     tables, bset, _, m = _valley_setup(backend)
     rain = Rain(tables, bset, uniform_storm(40.0 * storm_s / 1800.0, storm_s), np.ones(tables.grid.shape))
     m.set_depth(np.full(tables.grid.shape, 0.3))
-    Driver(m, bset, rain, maxima_every=1)
+    drv = Driver(m, bset, rain, maxima_every=1)
+    return drv, bset, rain, m
+
+
+def _wet_valley_run(backend, pipeline, t_end, chunks, storm_s):
+    _, bset, rain, m = _wet_valley(backend, storm_s)
     run = DeviceRun(m.backend, [_device_plan(m, bset, rain, t_end)], m.num, maxima=1, idle=m.stepper.idle,
                     pipeline=pipeline)  # fmt: skip
     for n, graph in chunks:
@@ -175,7 +180,8 @@ def test_window_ends_and_checks_keep_the_pipeline_when_the_list_cannot_change(ba
     either (each tile is kept whatever its wet flag says): the step before either leaves its update to the next step's
     pipe kernel like any other, and only each chunk's last step (whose update run_steps makes before it returns) and
     the run's last take another path. A storm that stops at 600 s turns the checks' rule off mid-run (and with it,
-    as tiles dry and leave, the windows'). Either way the two-kernel run bit for bit."""
+    as tiles dry and leave, the windows'). Either way the two-kernel run bit for bit, and the host loop's (whose
+    stepper checks the list on the host), to its last step."""
     t_end = 1500.0
     chunks = [(90, 16), (33, 0), (5000, 64)]
     a, sa = _wet_valley_run(backend, True, t_end, chunks, storm_s)
@@ -191,3 +197,15 @@ def test_window_ends_and_checks_keep_the_pipeline_when_the_list_cannot_change(ba
         assert sa.pipelined_updates < n - 5
     _same_state(sa, sb)
     _same_model(a, b, a.grid)
+    drv, _, _, host = _wet_valley(backend, storm_s)
+    drv.run(t_end)
+    c, sc = _wet_valley_run(backend, True, t_end, [(host.stepper.nsteps, 16)], storm_s)
+    assert sc.nsteps == host.stepper.nsteps and sc.t == host.stepper.t and sc.dt == host.stepper.dt
+    assert c.backend.tally_volume() == host.backend.tally_volume()
+    for f in FIELDS:
+        assert np.array_equal(c.field(f), host.field(f)), f
+    for which in range(5):
+        x = from_state(host.grid, host.backend.maxima_get(which))
+        y = from_state(c.grid, c.backend.maxima_get(which))
+        assert np.array_equal(x, y, equal_nan=True), which
+    assert np.array_equal(np.sort(c.backend.active_tiles), np.sort(host.backend.active_tiles))

@@ -2378,11 +2378,19 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
     Member& b = mem[blockIdx.x];
     int nw, nl;
     bool changed = false;
+    // with every tile of the member listed, a window's rebuild (which keeps what is listed) changes nothing, and so
+    // does a check when every tile is also kept whatever its wet flag says: it only sets each tile's idle count to 0
+    const int own = b.tile1 - b.tile0;
+    const bool listed = b.nlist == own;
     if (b.check) {
-        rebuild(b, 2, L, wave_cnt, nw, nl, sflags);
-        transitions<R, FAST>(b, nw, nl, L, B, hc, ctab, ncell, last, dx, fresh);
+        if (listed && !num.excess && (b.scale != 0.0 ? b.kept_rain : b.kept_dry) == own) {
+            for (int t = b.tile0 + threadIdx.x; t < b.tile1; t += PRE_THREADS) L.idle[t] = 0;
+        } else {
+            rebuild(b, 2, L, wave_cnt, nw, nl, sflags);
+            transitions<R, FAST>(b, nw, nl, L, B, hc, ctab, ncell, last, dx, fresh);
+            changed = true;
+        }
         if (threadIdx.x == 0) b.check = 0;
-        changed = true;
     }
     if (!flush_only) {
         // the window's rain in the excess form: the sum of the member's tile sums excess_kernel left
@@ -2407,7 +2415,7 @@ __global__ __launch_bounds__(PRE_THREADS) void pre_kernel(Member* __restrict__ m
             any = mine;
         }
         __syncthreads();
-        if (any) {
+        if (any && !(b.rebuild == 1 && b.nlist == own)) {
             rebuild(b, b.rebuild, L, wave_cnt, nw, nl, sflags);
             transitions<R, FAST>(b, nw, nl, L, B, hc, ctab, ncell, last, dx, fresh);
             changed = true;
