@@ -695,8 +695,8 @@ struct Member {
     // (`fuse`) leaves its cell update to the next step's pipe kernel; `pend` says an update waits there, with its
     // step's parameters (`u_*`: length, wave step, new window and scale, parity, steps before it, clock after it).
     int pipe, fuse, pend;
-    int kept_rain, kept_dry;  // the member's tiles a check keeps whatever their wet flags (a source while it rains, a
-                              // boundary cell)
+    int kept_rain, kept_dry;  // how many of the member's tiles a check keeps whatever their wet flags say: while it
+                              // rains (a source or a boundary cell in the tile) and when dry (a boundary cell)
     int u_want, u_new_window, u_par;
     long long u_nsteps;
     long long u_made;  // updates the pipe kernel has made for this member (a count for the host)
