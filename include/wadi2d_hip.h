@@ -208,10 +208,11 @@ int w2d_run_read(w2d_handle* h, w2d_member_state* out, int32_t M);
  * change nothing of it, whatever the other members do. */
 int w2d_run_pause(w2d_handle* h, const double* t_stop, int32_t M);
 /* The pipelined step of device-resident runs (no counterpart; on by default in FP32, off in FP64; taken by the run
- * set up, from its next step, and by the next w2d_run_setup): a
- * step whose next step keeps the active list and reads no wet flag leaves its cell update to that step's kernel,
- * which makes it and then the face step in one pass over each tile, recomputing the update of its west column and
- * north row instead of reading the level and limited discharges back from memory. Every field, maximum and book is
+ * set up, from its next step, and by the next w2d_run_setup): a step whose next step keeps the active list and reads
+ * no wet flag (no check or window end follows, or one that cannot change the list: every tile listed, and for a
+ * check every tile kept whatever its wet flag) leaves its cell update to that step's kernel, which makes it and then
+ * the face step in one pass over each tile, recomputing the update of its west column and north row instead of
+ * reading the level and limited discharges back from memory. Every field, maximum and book is
  * the two-kernel step's, bit for bit; run_steps returns with no update waiting. Runs with the learned closure, and
  * members with a following boundary cell (kind 2) whose reference lies outside what a tile updates, take the
  * two-kernel step. on: 0 or 1. */
